@@ -1,0 +1,415 @@
+// C ABI of the MI355X DGSEM right-hand side (include/dgsem_amd.h).
+//
+// The handle owns the device copies of the geometry, the neighbour table,
+// the basis tables and the per-element blending factor; state arrays belong
+// to the caller.  Everything is enqueued on the handle's stream, with no
+// allocation, host synchronisation or host<->device traffic inside dgsem_rhs
+// and dgsem_ssprk3_stage, so both can be captured in a HIP graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/dgsem_amd.h"
+#include "host_setup.hpp"
+#include "kernels.hpp"
+
+struct dgsem_handle {
+  dgsem_config cfg{};
+  dgsem::BasisTables basis;
+  dgsem::Geometry geo;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int nvars = 0, nn = 0;
+  int64_t E = 0;
+  // device buffers
+  double* d_Ja = nullptr;
+  double* d_invJ = nullptr;
+  double* d_sn = nullptr;
+  int* d_nbr = nullptr;
+  double* d_Vinv = nullptr;
+  double* d_alpha = nullptr;
+  dgsem::KParams base{};
+  std::string err;
+};
+
+namespace {
+
+int fail(dgsem_handle* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+int hip_fail(dgsem_handle* h, hipError_t e, const char* where) {
+  return fail(h, DGSEM_E_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(h, expr)                                \
+  do {                                                  \
+    hipError_t _e = (expr);                             \
+    if (_e != hipSuccess) return hip_fail(h, _e, #expr); \
+  } while (0)
+
+template <class T>
+int upload(dgsem_handle* h, T** dst, const std::vector<T>& src) {
+  const size_t bytes = src.size() * sizeof(T);
+  hipError_t e = hipMalloc((void**)dst, bytes ? bytes : sizeof(T));
+  if (e != hipSuccess) return fail(h, DGSEM_E_ALLOC, "hipMalloc failed");
+  if (bytes) {
+    e = hipMemcpy(*dst, src.data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(h, e, "hipMemcpy");
+  }
+  return DGSEM_OK;
+}
+
+void release(dgsem_handle* h) {
+  hipFree(h->d_Ja);
+  hipFree(h->d_invJ);
+  hipFree(h->d_sn);
+  hipFree(h->d_nbr);
+  hipFree(h->d_Vinv);
+  hipFree(h->d_alpha);
+  h->d_Ja = h->d_invJ = h->d_sn = h->d_Vinv = h->d_alpha = nullptr;
+  h->d_nbr = nullptr;
+}
+
+// Geometry in the kernels' layouts.  Nodal: Ja[E][dir][comp][nn] and
+// invJ[E][nn] copied from the reference-layout arrays.  Affine: the exact
+// metric terms of the linear map of each element.
+void device_geometry(const dgsem_handle* h, std::vector<double>& Ja, std::vector<double>& invJ) {
+  const auto& g = h->geo;
+  const int d = g.dim, nn = g.nn;
+  if (h->cfg.geometry == DGSEM_GEOMETRY_NODAL) {
+    Ja.assign(g.E * d * d * nn, 0.0);
+    for (int64_t e = 0; e < g.E; ++e)
+      for (int dof = 0; dof < nn; ++dof)
+        for (int r = 0; r < d; ++r)
+          for (int c = 0; c < d; ++c)
+            Ja[((e * d + r) * d + c) * nn + dof] = g.contravariant[((e * nn + dof) * d + r) * d + c];
+    invJ = g.inverse_jacobian;
+  } else {
+    // x = L + (xi_cell + 1)/2 * h: dx/dxi = h/2.  Ja^r_c = delta_rc * prod_{k != r} h_k/2,
+    // 1/J = prod_k 2/h_k.
+    double half[3] = {1.0, 1.0, 1.0};
+    for (int k = 0; k < d; ++k)
+      half[k] = 0.5 * (h->cfg.domain_right[k] - h->cfg.domain_left[k]) / (double)h->cfg.cells[k];
+    double ja[3][3] = {{0}};
+    double det = 1.0;
+    for (int k = 0; k < d; ++k) det *= half[k];
+    for (int r = 0; r < d; ++r) {
+      double prod = 1.0;
+      for (int k = 0; k < d; ++k)
+        if (k != r) prod *= half[k];
+      ja[r][r] = prod;
+    }
+    Ja.assign(g.E * d * d, 0.0);
+    invJ.assign(g.E, 1.0 / det);
+    for (int64_t e = 0; e < g.E; ++e)
+      for (int r = 0; r < d; ++r)
+        for (int c = 0; c < d; ++c) Ja[(e * d + r) * d + c] = ja[r][c];
+  }
+}
+
+// (lower, upper) neighbour per direction; periodic wrap.
+std::vector<int> neighbour_table(const dgsem::Geometry& g) {
+  const int d = g.dim;
+  std::vector<int> nb(g.E * 2 * d);
+  for (int ie = 0; ie < g.cells[0]; ++ie)
+    for (int je = 0; je < g.cells[1]; ++je)
+      for (int ke = 0; ke < g.cells[2]; ++ke) {
+        const int64_t e = ((int64_t)ie * g.cells[1] + je) * g.cells[2] + ke;
+        const int id[3] = {ie, je, ke};
+        for (int r = 0; r < d; ++r)
+          for (int side = 0; side < 2; ++side) {
+            int o[3] = {id[0], id[1], id[2]};
+            o[r] = (o[r] + (side ? 1 : -1) + g.cells[r]) % g.cells[r];
+            nb[e * 2 * d + 2 * r + side] = (int)(((int64_t)o[0] * g.cells[1] + o[1]) * g.cells[2] + o[2]);
+          }
+      }
+  return nb;
+}
+
+void fill_params(dgsem_handle* h) {
+  dgsem::KParams& p = h->base;
+  const auto& c = h->cfg;
+  const auto& b = h->basis;
+  std::memset(&p, 0, sizeof(p));
+  p.Ja = h->d_Ja;
+  p.invJ = h->d_invJ;
+  p.sn = h->d_sn;
+  p.nbr = h->d_nbr;
+  p.Vinv = h->d_Vinv;
+  p.alpha = h->d_alpha;
+  p.E = h->E;
+  p.surface_flux = c.surface_flux;
+  p.fv_flux = c.fv_flux;
+  p.gamma = c.gamma;
+  p.inv_gm1 = 1.0 / (c.gamma - 1.0);
+  p.alpha_max = c.alpha_max;
+  p.alpha_min = c.alpha_min;
+  // atol = max(100 eps, eps^0.75), volume_flux.hpp:772-773
+  const double eps = std::numeric_limits<double>::epsilon();
+  p.atol = std::max(100.0 * eps, std::pow(eps, 0.75));
+  // threshold and sharpness, indicator.hpp:443-452
+  p.threshold = 0.5 * std::pow(10.0, -1.8 * std::pow((double)b.n, 0.25));
+  p.sfac = std::log((1.0 - 0.0001) / 0.0001);
+  p.floor_value = c.positivity_floor;
+  p.lhat0 = b.lhat_left[0];
+  p.lhatN = b.lhat_right[b.n - 1];
+  for (int i = 0; i < b.n; ++i) {
+    p.inv_w[i] = b.inv_weights[i];
+    for (int j = 0; j < b.n; ++j) p.Ds[i][j] = b.Dsplit[i * b.n + j];
+  }
+}
+
+int check_config(dgsem_handle* h, const dgsem_config& c) {
+  if (c.dim != 2 && c.dim != 3) return fail(h, DGSEM_E_INVALID, "dim must be 2 or 3");
+  if (c.polydeg < 1 || c.polydeg > 7) return fail(h, DGSEM_E_INVALID, "polydeg must be 1..7");
+  for (int k = 0; k < c.dim; ++k) {
+    if (c.cells[k] < 1) return fail(h, DGSEM_E_INVALID, "cells must be positive");
+    if (!(c.domain_right[k] > c.domain_left[k]))
+      return fail(h, DGSEM_E_INVALID, "domain_right must exceed domain_left");
+    if (c.boundary[k] != DGSEM_BC_PERIODIC)
+      return fail(h, DGSEM_E_UNSUPPORTED, "only periodic boundaries are built");
+  }
+  const int64_t E = (int64_t)c.cells[0] * c.cells[1] * (c.dim == 3 ? c.cells[2] : 1);
+  if (E > std::numeric_limits<int>::max() / 2) return fail(h, DGSEM_E_INVALID, "mesh too large");
+  if (c.equation != DGSEM_EULER)
+    return fail(h, DGSEM_E_UNSUPPORTED, "Navier-Stokes terms are not built in this library");
+  if (c.volume_integral != DGSEM_VOLUME_SPLIT_FORM && c.volume_integral != DGSEM_VOLUME_SHOCK_CAPTURING)
+    return fail(h, DGSEM_E_INVALID, "unknown volume integral");
+  if (c.volume_flux != DGSEM_FLUX_CHANDRASHEKAR) return fail(h, DGSEM_E_INVALID, "unknown volume flux");
+  if ((c.surface_flux != DGSEM_FLUX_LLF && c.surface_flux != DGSEM_FLUX_HLLC) ||
+      (c.fv_flux != DGSEM_FLUX_LLF && c.fv_flux != DGSEM_FLUX_HLLC))
+    return fail(h, DGSEM_E_INVALID, "unknown surface or FV flux");
+  if (c.geometry != DGSEM_GEOMETRY_NODAL && c.geometry != DGSEM_GEOMETRY_AFFINE)
+    return fail(h, DGSEM_E_INVALID, "unknown geometry mode");
+  if (!(c.gamma > 1.0)) return fail(h, DGSEM_E_INVALID, "gamma must exceed 1");
+  if (c.dim == 3 && c.polydeg == 1) return fail(h, DGSEM_E_UNSUPPORTED, "3D N=1 is not built");
+  if (c.polydeg == 6) return fail(h, DGSEM_E_UNSUPPORTED, "N=6 is not built");
+  return DGSEM_OK;
+}
+
+hipStream_t stream_of(const dgsem_handle* h) { return h->stream; }
+
+int launch(dgsem_handle* h, const dgsem::KParams& p) {
+  const bool sc = h->cfg.volume_integral == DGSEM_VOLUME_SHOCK_CAPTURING;
+  const bool nodal = h->cfg.geometry == DGSEM_GEOMETRY_NODAL;
+  hipError_t e = dgsem::launch_rhs(h->cfg.dim, h->cfg.polydeg, sc, nodal, p, stream_of(h));
+  if (e != hipSuccess) return hip_fail(h, e, "rhs kernel launch");
+  return DGSEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dgsem_abi_version(void) { return DGSEM_ABI_VERSION; }
+
+void dgsem_config_default(dgsem_config* c, int dim, int polydeg) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->dim = dim;
+  c->polydeg = polydeg;
+  for (int k = 0; k < 3; ++k) {
+    c->cells[k] = 1;
+    c->domain_left[k] = -1.0;
+    c->domain_right[k] = 1.0;
+    c->boundary[k] = DGSEM_BC_PERIODIC;
+  }
+  c->equation = DGSEM_EULER;
+  c->gamma = 1.4;
+  c->mu = dim == 3 ? 6.25e-4 : 5.0e-3;  // compressible_navier_stokes{3D,2D}.hpp defaults
+  c->prandtl = dim == 3 ? 0.72 : 0.73;
+  c->volume_integral = DGSEM_VOLUME_SPLIT_FORM;
+  c->volume_flux = DGSEM_FLUX_CHANDRASHEKAR;
+  c->surface_flux = DGSEM_FLUX_LLF;
+  c->fv_flux = DGSEM_FLUX_LLF;
+  c->alpha_max = 0.5;  // structured_solver.hpp:140-142
+  c->alpha_min = 0.001;
+  c->alpha_smooth = 0;
+  c->positivity_floor = 0.0;
+  c->geometry = DGSEM_GEOMETRY_NODAL;
+}
+
+int dgsem_create(const dgsem_config* cfg, void* stream, dgsem_handle** out) {
+  if (!cfg || !out) return DGSEM_E_INVALID;
+  *out = nullptr;
+  dgsem_handle* h = new (std::nothrow) dgsem_handle();
+  if (!h) return DGSEM_E_ALLOC;
+  h->cfg = *cfg;
+  if (h->cfg.dim == 2) h->cfg.cells[2] = 1;
+  int rc = check_config(h, h->cfg);
+  if (rc != DGSEM_OK) {
+    // keep the handle so the caller can read the message
+    *out = h;
+    return rc;
+  }
+  h->stream = (hipStream_t)stream;
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    *out = h;
+    return fail(h, DGSEM_E_HIP, "no HIP device");
+  }
+  try {
+    h->basis = dgsem::make_basis(h->cfg.polydeg);
+    const int periodic[3] = {1, 1, 1};  // check_config admits periodic boundaries only
+    h->geo = dgsem::make_geometry(h->basis, h->cfg.dim, h->cfg.cells, h->cfg.domain_left,
+                                  h->cfg.domain_right, periodic);
+  } catch (const std::exception& ex) {
+    *out = h;
+    return fail(h, DGSEM_E_INVALID, ex.what());
+  }
+  h->nvars = h->cfg.dim + 2;
+  h->nn = h->geo.nn;
+  h->E = h->geo.E;
+  std::vector<double> Ja, invJ;
+  device_geometry(h, Ja, invJ);
+  std::vector<double> sn;
+  if (h->cfg.geometry == DGSEM_GEOMETRY_NODAL && h->cfg.volume_integral == DGSEM_VOLUME_SHOCK_CAPTURING)
+    sn = h->geo.subcell_normals;
+  if ((rc = upload(h, &h->d_Ja, Ja)) || (rc = upload(h, &h->d_invJ, invJ)) ||
+      (rc = upload(h, &h->d_sn, sn)) || (rc = upload(h, &h->d_nbr, neighbour_table(h->geo))) ||
+      (rc = upload(h, &h->d_Vinv, h->basis.Vinv)) ||
+      (rc = upload(h, &h->d_alpha, std::vector<double>(h->E, 0.0)))) {
+    release(h);
+    *out = h;
+    return rc;
+  }
+  fill_params(h);
+  *out = h;
+  return DGSEM_OK;
+}
+
+int dgsem_destroy(dgsem_handle* h) {
+  if (!h) return DGSEM_OK;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  else hipDeviceSynchronize();
+  release(h);
+  delete h;
+  return DGSEM_OK;
+}
+
+const char* dgsem_last_error(const dgsem_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int dgsem_sizes(const dgsem_handle* h, int64_t* E, int* ndofs, int* nvars) {
+  if (!h || !h->d_Ja) return DGSEM_E_INVALID;
+  if (E) *E = h->E;
+  if (ndofs) *ndofs = h->nn;
+  if (nvars) *nvars = h->nvars;
+  return DGSEM_OK;
+}
+
+int dgsem_set_indicator_parameters(dgsem_handle* h, double amax, double amin, int smooth) {
+  if (!h || !h->d_Ja) return DGSEM_E_INVALID;
+  h->cfg.alpha_max = amax;
+  h->cfg.alpha_min = amin;
+  h->cfg.alpha_smooth = smooth;
+  h->base.alpha_max = amax;
+  h->base.alpha_min = amin;
+  return DGSEM_OK;
+}
+
+int dgsem_node_coordinates(const dgsem_handle* h, double* out) {
+  if (!h || !h->d_Ja || !out) return DGSEM_E_INVALID;
+  std::memcpy(out, h->geo.coords.data(), h->geo.coords.size() * sizeof(double));
+  return DGSEM_OK;
+}
+
+int dgsem_contravariant_vectors(const dgsem_handle* h, double* out) {
+  if (!h || !h->d_Ja || !out) return DGSEM_E_INVALID;
+  std::memcpy(out, h->geo.contravariant.data(), h->geo.contravariant.size() * sizeof(double));
+  return DGSEM_OK;
+}
+
+int dgsem_inverse_jacobian(const dgsem_handle* h, double* out) {
+  if (!h || !h->d_Ja || !out) return DGSEM_E_INVALID;
+  std::memcpy(out, h->geo.inverse_jacobian.data(), h->geo.inverse_jacobian.size() * sizeof(double));
+  return DGSEM_OK;
+}
+
+int dgsem_pack_state(dgsem_handle* h, const double* ref, double* internal) {
+  if (!h || !h->d_Ja || !ref || !internal) return DGSEM_E_INVALID;
+  hipError_t e = dgsem::launch_pack(ref, internal, h->E, h->nn, h->nvars, true, h->stream);
+  return e == hipSuccess ? DGSEM_OK : hip_fail(h, e, "pack");
+}
+
+int dgsem_unpack_state(dgsem_handle* h, const double* internal, double* ref) {
+  if (!h || !h->d_Ja || !ref || !internal) return DGSEM_E_INVALID;
+  hipError_t e = dgsem::launch_pack(internal, ref, h->E, h->nn, h->nvars, false, h->stream);
+  return e == hipSuccess ? DGSEM_OK : hip_fail(h, e, "unpack");
+}
+
+int dgsem_rhs(dgsem_handle* h, const double* u, double* du, double /*t*/) {
+  if (!h || !h->d_Ja) return DGSEM_E_INVALID;
+  if (!u || !du || u == du) return fail(h, DGSEM_E_INVALID, "u and du must be distinct device arrays");
+  dgsem::KParams p = h->base;
+  p.u = u;
+  p.out = du;
+  p.mode = dgsem::kModeRhs;
+  return launch(h, p);
+}
+
+int dgsem_get_alpha(const dgsem_handle* h, double* alpha) {
+  if (!h || !h->d_Ja || !alpha) return DGSEM_E_INVALID;
+  hipError_t e = hipMemcpyAsync(alpha, h->d_alpha, h->E * sizeof(double), hipMemcpyDeviceToDevice,
+                                h->stream);
+  return e == hipSuccess ? DGSEM_OK : DGSEM_E_HIP;
+}
+
+int dgsem_ssprk3_stage(dgsem_handle* h, int stage, double dt, const double* u_base,
+                       const double* u_k, double* u_out) {
+  if (!h || !h->d_Ja) return DGSEM_E_INVALID;
+  if (!u_k || !u_out || u_out == u_k) return fail(h, DGSEM_E_INVALID, "u_out must not alias u_k");
+  dgsem::KParams p = h->base;
+  p.u = u_k;
+  p.out = u_out;
+  // coefficients exactly as SSPRK3::step builds them, SSPRK3.hpp:162-180
+  if (stage == 1) {
+    p.mode = dgsem::kModeStage2;
+    p.c1 = 1.0;
+    p.c2 = dt;
+  } else if (stage == 2 || stage == 3) {
+    if (!u_base) return fail(h, DGSEM_E_INVALID, "u_base required for stages 2 and 3");
+    p.mode = dgsem::kModeStage3;
+    p.u_base = u_base;
+    if (stage == 2) {
+      p.c1 = 3.0 / 4.0;
+      p.c2 = 1.0 / 4.0;
+      p.c3 = 1.0 / 4.0 * dt;
+    } else {
+      p.c1 = 1.0 / 3.0;
+      p.c2 = 2.0 / 3.0;
+      p.c3 = 2.0 / 3.0 * dt;
+    }
+  } else {
+    return fail(h, DGSEM_E_INVALID, "stage must be 1, 2 or 3");
+  }
+  return launch(h, p);
+}
+
+int dgsem_ssprk3_steps(dgsem_handle* h, double* u, double dt, int nsteps, double* tmp1, double* tmp2) {
+  if (!h || !h->d_Ja) return DGSEM_E_INVALID;
+  if (!u || !tmp1 || !tmp2 || u == tmp1 || u == tmp2 || tmp1 == tmp2)
+    return fail(h, DGSEM_E_INVALID, "u, tmp1, tmp2 must be distinct device arrays");
+  for (int s = 0; s < nsteps; ++s) {
+    int rc;
+    if ((rc = dgsem_ssprk3_stage(h, 1, dt, nullptr, u, tmp1))) return rc;
+    if ((rc = dgsem_ssprk3_stage(h, 2, dt, u, tmp1, tmp2))) return rc;
+    if ((rc = dgsem_ssprk3_stage(h, 3, dt, u, tmp2, u))) return rc;
+  }
+  return DGSEM_OK;
+}
+
+int dgsem_synchronize(dgsem_handle* h) {
+  if (!h) return DGSEM_E_INVALID;
+  hipError_t e = h->stream ? hipStreamSynchronize(h->stream) : hipDeviceSynchronize();
+  return e == hipSuccess ? DGSEM_OK : hip_fail(h, e, "synchronize");
+}
+
+}  // extern "C"

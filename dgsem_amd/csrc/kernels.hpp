@@ -1,0 +1,51 @@
+// Kernel parameter block and launch entry points shared by the ABI layer
+// and the kernel translation unit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dgsem {
+
+constexpr int kMaxN1 = 8;  // max nodes per direction (N <= 7)
+
+enum StageMode : int {
+  kModeRhs = 0,     // out = L(u)
+  kModeStage2 = 1,  // out = c1 u + c2 L(u)                (parallel_ma2)
+  kModeStage3 = 2,  // out = c1 u_base + c2 u + c3 L(u)    (parallel_ma3)
+};
+
+struct KParams {
+  const double* u;       // stage input, [E][NV][NN]
+  const double* u_base;  // base state for kModeStage3
+  double* out;           // L(u) or the stage result
+  double* alpha;         // [E] blending factor (shock capturing), written
+  const double* Ja;      // nodal: [E][dim][dim][NN]; affine: [E][dim][dim]
+  const double* invJ;    // nodal: [E][NN]; affine: [E]
+  const double* sn;      // nodal subcell normals [E][dim][FN][N][dim]
+  const int* nbr;        // [E][2*dim] (lower, upper) neighbour per direction
+  const double* Vinv;    // [n][n] inverse Legendre Vandermonde
+  int64_t E;
+  int mode;
+  int surface_flux, fv_flux;
+  double c1, c2, c3;
+  double gamma, inv_gm1;
+  double alpha_max, alpha_min, atol, threshold, sfac;
+  double floor_value;
+  double lhat0, lhatN;
+  double Ds[kMaxN1][kMaxN1];
+  double inv_w[kMaxN1];
+};
+
+// Launches the fused RHS / stage kernel for the configuration; returns
+// hipErrorInvalidValue when (dim, N, shock capturing, nodal) is not built.
+hipError_t launch_rhs(int dim, int N, bool shock_capturing, bool nodal, const KParams& p,
+                      hipStream_t stream);
+
+// Layout conversion between the reference layout [e][dof][v] and the
+// internal layout [e][v][dof]; to_internal selects the direction.
+hipError_t launch_pack(const double* src, double* dst, int64_t E, int nn, int nv,
+                       bool to_internal, hipStream_t stream);
+
+}  // namespace dgsem

@@ -1,0 +1,229 @@
+// Pointwise physics for the compressible Euler / Navier-Stokes equations in
+// 2D and 3D: physical flux, LLF and HLLC surface fluxes, the Chandrashekar
+// entropy-conservative / kinetic-energy-preserving two-point flux and the
+// logarithmic mean.  Expressions keep the reference's evaluation order so the
+// results agree with it to rounding.
+//
+// Reference: src/base/math_utils.hpp:8-24 (ln_mean),
+// src/equations/compressible_euler{2D,3D}.hpp (flux, wave speed),
+// src/base/numerical_flux.hpp:39-135,198-228 (LLF), :436-594 (Chandrashekar),
+// src/base/numerical_flux_detail.hpp:9-21.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace dgsem {
+
+enum FluxKind : int { kFluxLLF = 0, kFluxHLLC = 1 };
+
+#define DG_HD __host__ __device__ __forceinline__
+
+DG_HD double dmax(double a, double b) { return (a < b) ? b : a; }
+DG_HD double dmin(double a, double b) { return (b < a) ? b : a; }
+
+// Logarithmic mean (y - x) / log(y / x) with the series branch for
+// f2 = ((x - y) / (x + y))^2 < 1e-4, math_utils.hpp:8-24.
+DG_HD double ln_mean(double x, double y) {
+  const double num = x * (x - 2.0 * y) + y * y;
+  const double den = x * (x + 2.0 * y) + y * y;
+  const double f2 = num / den;
+  if (f2 < 1.0e-4) {
+    const double poly = 2.0 + f2 * (2.0 / 3.0 + f2 * (2.0 / 5.0 + f2 * (2.0 / 7.0)));
+    return (x + y) / poly;
+  }
+  return (y - x) / log(y / x);
+}
+
+// Pressure of a conservative state.
+template <int DIM>
+DG_HD double pressure(double gamma, const double* u) {
+  double v2 = 0.0;
+  for (int k = 0; k < DIM; ++k) {
+    const double v = u[k + 1] / u[0];
+    v2 += v * v;
+  }
+  return (gamma - 1.0) * (u[DIM + 1] - 0.5 * u[0] * v2);
+}
+
+// Physical flux F(u) . n: compressible_euler2D.hpp:25-57,
+// compressible_euler3D.hpp:26-67 (sum of the per-direction fluxes).
+template <int DIM>
+DG_HD void physical_flux_n(double gamma, const double* u, const double* n, double* f) {
+  constexpr int NV = DIM + 2;
+  const double rho = u[0], rhoE = u[NV - 1];
+  double v[DIM];
+  double v2 = 0.0;
+  for (int k = 0; k < DIM; ++k) v[k] = u[k + 1] / rho;
+  for (int k = 0; k < DIM; ++k) v2 += v[k] * v[k];
+  const double p = (gamma - 1.0) * (rhoE - 0.5 * rho * v2);
+  for (int i = 0; i < NV; ++i) f[i] = 0.0;
+  for (int d = 0; d < DIM; ++d) {
+    // flux in direction d: (m_d, m_d v + p e_d, v_d (E + p))
+    double fd[NV];
+    fd[0] = u[d + 1];
+    for (int k = 0; k < DIM; ++k) fd[k + 1] = u[d + 1] * v[k] + (k == d ? p : 0.0);
+    fd[NV - 1] = v[d] * (rhoE + p);
+    for (int i = 0; i < NV; ++i) f[i] = f[i] + n[d] * fd[i];
+  }
+}
+
+// max(|v_L.n|, |v_R.n|) + max(c_L, c_R) |n|, compressible_euler3D.hpp:121-166.
+template <int DIM>
+DG_HD double llf_wave_speed(double gamma, const double* ul, const double* ur, const double* n) {
+  double vnl = 0.0, vnr = 0.0, nn = 0.0;
+  for (int k = 0; k < DIM; ++k) {
+    vnl += (ul[k + 1] / ul[0]) * n[k];
+    vnr += (ur[k + 1] / ur[0]) * n[k];
+    nn += n[k] * n[k];
+  }
+  const double al = sqrt(gamma * pressure<DIM>(gamma, ul) / ul[0]);
+  const double ar = sqrt(gamma * pressure<DIM>(gamma, ur) / ur[0]);
+  return dmax(fabs(vnl), fabs(vnr)) + dmax(al, ar) * sqrt(nn);
+}
+
+// Local Lax-Friedrichs: 0.5 (F_L + F_R).n - 0.5 lambda (u_R - u_L).
+template <int DIM>
+DG_HD void flux_llf(double gamma, const double* ul, const double* ur, const double* n, double* f) {
+  constexpr int NV = DIM + 2;
+  double fl[NV], fr[NV];
+  const double lam = llf_wave_speed<DIM>(gamma, ul, ur, n);
+  physical_flux_n<DIM>(gamma, ul, n, fl);
+  physical_flux_n<DIM>(gamma, ur, n, fr);
+  for (int i = 0; i < NV; ++i) f[i] = 0.5 * (fl[i] + fr[i]) - 0.5 * lam * (ur[i] - ul[i]);
+}
+
+// HLLC with a normal vector.  The reference has HLLC for 1D Euler only
+// (numerical_flux.hpp:283-378); this applies the same construction
+// (Roe-averaged wave-speed bounds, contact speed, star states) in the frame of
+// the unit normal with the tangential velocity carried through, and scales
+// the result by |n|.
+template <int DIM>
+DG_HD void flux_hllc(double gamma, const double* ul, const double* ur, const double* n, double* f) {
+  constexpr int NV = DIM + 2;
+  double nrm = 0.0, nh[DIM], vl[DIM], vr[DIM], fl[NV], fr[NV];
+  for (int d = 0; d < DIM; ++d) nrm += n[d] * n[d];
+  nrm = sqrt(nrm);
+  for (int d = 0; d < DIM; ++d) nh[d] = n[d] / nrm;
+  const double rl = ul[0], rr = ur[0], El = ul[NV - 1], Er = ur[NV - 1];
+  double v2l = 0.0, v2r = 0.0, vnl = 0.0, vnr = 0.0;
+  for (int d = 0; d < DIM; ++d) {
+    vl[d] = ul[d + 1] / rl;
+    vr[d] = ur[d + 1] / rr;
+    v2l += vl[d] * vl[d];
+    v2r += vr[d] * vr[d];
+    vnl += vl[d] * nh[d];
+    vnr += vr[d] * nh[d];
+  }
+  const double el = El / rl, er = Er / rr;
+  const double pl = (gamma - 1.0) * (El - 0.5 * rl * v2l);
+  const double pr = (gamma - 1.0) * (Er - 0.5 * rr * v2r);
+  const double cl = sqrt(gamma * pl / rl), cr = sqrt(gamma * pr / rr);
+  physical_flux_n<DIM>(gamma, ul, nh, fl);
+  physical_flux_n<DIM>(gamma, ur, nh, fr);
+  const double sl = sqrt(rl), sr = sqrt(rr), ss = sl + sr;
+  double vroe2 = 0.0;
+  for (int d = 0; d < DIM; ++d) {
+    const double vd = (sl * vl[d] + sr * vr[d]) / ss;
+    vroe2 += vd * vd;
+  }
+  const double vroe = (sl * vnl + sr * vnr) / ss;
+  const double Hl = (El + pl) / rl, Hr = (Er + pr) / rr;
+  const double Hroe = (sl * Hl + sr * Hr) / ss;
+  const double croe = sqrt((gamma - 1.0) * (Hroe - 0.5 * vroe2));
+  const double SL = dmin(vnl - cl, vroe - croe);
+  const double SR = dmax(vnr + cr, vroe + croe);
+  if (SL >= 0) {
+    for (int i = 0; i < NV; ++i) f[i] = fl[i];
+  } else if (SR <= 0) {
+    for (int i = 0; i < NV; ++i) f[i] = fr[i];
+  } else {
+    const double mL = SL - vnl, mR = SR - vnr;
+    const double Ss = (pr - pl + rl * vnl * mL - rr * vnr * mR) / (rl * mL - rr * mR);
+    const bool left = Ss >= 0;
+    const double* uk = left ? ul : ur;
+    const double* fk = left ? fl : fr;
+    const double* vk = left ? vl : vr;
+    const double rk = left ? rl : rr, vnk = left ? vnl : vnr, ek = left ? el : er;
+    const double pk = left ? pl : pr, Sk = left ? SL : SR, mk = left ? mL : mR;
+    const double rs = rk * mk / (Sk - Ss);
+    const double es = ek + (Ss - vnk) * (Ss + pk / (rk * mk));
+    f[0] = fk[0] + Sk * (rs - uk[0]);
+    for (int d = 0; d < DIM; ++d) f[d + 1] = fk[d + 1] + Sk * (rs * (vk[d] + (Ss - vnk) * nh[d]) - uk[d + 1]);
+    f[NV - 1] = fk[NV - 1] + Sk * (rs * es - uk[NV - 1]);
+  }
+  for (int i = 0; i < NV; ++i) f[i] *= nrm;
+}
+
+template <int DIM>
+DG_HD void surface_flux(int kind, double gamma, const double* ul, const double* ur, const double* n,
+                        double* f) {
+  if (kind == kFluxHLLC)
+    flux_hllc<DIM>(gamma, ul, ur, n, f);
+  else
+    flux_llf<DIM>(gamma, ul, ur, n, f);
+}
+
+// Per-node quantities hoisted out of the two-point flux: rho, v[DIM],
+// beta = rho / (2 p) and the specific kinetic energy |v|^2 / 2.
+template <int DIM>
+struct NodePrims {
+  double rho, v[DIM], beta, kin;
+};
+
+template <int DIM>
+DG_HD NodePrims<DIM> node_prims(double gamma, const double* u) {
+  NodePrims<DIM> q;
+  q.rho = u[0];
+  double v2 = 0.0;
+  for (int k = 0; k < DIM; ++k) q.v[k] = u[k + 1] / u[0];
+  for (int k = 0; k < DIM; ++k) v2 += q.v[k] * q.v[k];
+  const double p = (gamma - 1.0) * (u[DIM + 1] - 0.5 * u[0] * v2);
+  q.beta = 0.5 * u[0] / p;
+  q.kin = 0.5 * v2;
+  return q;
+}
+
+// Chandrashekar two-point flux from hoisted node quantities,
+// numerical_flux.hpp:436-500 (2D) and :520-594 (3D).  inv_gm1 = 1/(gamma-1).
+template <int DIM>
+DG_HD void flux_chandrashekar(double inv_gm1, const NodePrims<DIM>& a, const NodePrims<DIM>& b,
+                              const double* n, double* f) {
+  double vna = 0.0, vnb = 0.0;
+  for (int k = 0; k < DIM; ++k) {
+    vna += a.v[k] * n[k];
+    vnb += b.v[k] * n[k];
+  }
+  const double rho_avg = 0.5 * (a.rho + b.rho);
+  const double rho_mean = ln_mean(a.rho, b.rho);
+  const double beta_mean = ln_mean(a.beta, b.beta);
+  const double beta_avg = 0.5 * (a.beta + b.beta);
+  const double p_mean = 0.5 * rho_avg / beta_avg;
+  const double vsq_avg = a.kin + b.kin;
+  const double f1 = rho_mean * 0.5 * (vna + vnb);
+  double fe = f1 * 0.5 * (inv_gm1 / beta_mean - vsq_avg);
+  f[0] = f1;
+  for (int k = 0; k < DIM; ++k) {
+    const double vavg = 0.5 * (a.v[k] + b.v[k]);
+    f[k + 1] = f1 * vavg + p_mean * n[k];
+    fe = fe + f[k + 1] * vavg;
+  }
+  f[DIM + 1] = fe;
+}
+
+// p * rho with gamma fixed at 1.4, the indicator variable of
+// subcell_limiting/indicator.hpp:140-215.
+template <int DIM>
+DG_HD double indicator_variable(const double* u) {
+  const double g = 1.4;
+  double v2 = 0.0;
+  for (int k = 0; k < DIM; ++k) {
+    const double v = u[k + 1] / u[0];
+    v2 += v * v;
+  }
+  const double p = (g - 1.0) * (u[DIM + 1] - 0.5 * u[0] * v2);
+  return p * u[0];
+}
+
+}  // namespace dgsem

@@ -1,0 +1,192 @@
+"""Python mirror of the reference's solver interface over the HIP C ABI.
+
+StructuredSolver mirrors DGSEM::StructuredSolver (structured_solver.hpp:19-125):
+it is built from the equation, basis degree, volume/surface flux policies and
+the mesh, and exposes calc_rhs, set_indicator_parameters and get_ndofs.
+SSPRK3 mirrors DGSEM::SSPRK3 (SSPRK3.hpp:145-183).  Device memory and streams
+come from PyTorch; all arithmetic runs in the HIP kernels of libdgsem_amd.so.
+There is no CPU fallback: without a GPU or without the library these raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Sequence
+
+import numpy as np
+
+from . import _abi
+
+try:  # torch provides device memory and the stream
+    import torch
+except ImportError:  # pragma: no cover
+    torch = None
+
+
+@dataclass
+class SolverConfig:
+    dim: int
+    polydeg: int
+    cells: Sequence[int]
+    domain_left: Sequence[float]
+    domain_right: Sequence[float]
+    equation: int = _abi.EULER
+    gamma: float = 1.4
+    mu: float | None = None
+    prandtl: float | None = None
+    volume_integral: int = _abi.SPLIT_FORM
+    surface_flux: int = _abi.FLUX_LLF
+    fv_flux: int = _abi.FLUX_LLF
+    alpha_max: float = 0.5
+    alpha_min: float = 0.001
+    alpha_smooth: bool = False
+    positivity_floor: float = 0.0
+    geometry: int = _abi.GEOMETRY_NODAL
+
+    def to_c(self) -> _abi.Config:
+        L = _abi.load()
+        c = _abi.Config()
+        L.dgsem_config_default(C.byref(c), self.dim, self.polydeg)
+        for k in range(self.dim):
+            c.cells[k] = int(self.cells[k])
+            c.domain_left[k] = float(self.domain_left[k])
+            c.domain_right[k] = float(self.domain_right[k])
+        c.equation = self.equation
+        c.gamma = self.gamma
+        if self.mu is not None:
+            c.mu = self.mu
+        if self.prandtl is not None:
+            c.prandtl = self.prandtl
+        c.volume_integral = self.volume_integral
+        c.surface_flux = self.surface_flux
+        c.fv_flux = self.fv_flux
+        c.alpha_max = self.alpha_max
+        c.alpha_min = self.alpha_min
+        c.alpha_smooth = int(self.alpha_smooth)
+        c.positivity_floor = self.positivity_floor
+        c.geometry = self.geometry
+        return c
+
+
+def _ptr(t) -> int:
+    if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+        raise ValueError("expected a contiguous float64 tensor on the GPU")
+    return t.data_ptr()
+
+
+class StructuredSolver:
+    """DGSEM right-hand side on one GPU (internal state layout [E][var][dof])."""
+
+    def __init__(self, cfg: SolverConfig, stream=None):
+        if torch is None or not torch.cuda.is_available():
+            raise _abi.DgsemError("a ROCm GPU is required")
+        self.cfg = cfg
+        self._L = _abi.load()
+        self.stream = stream if stream is not None else torch.cuda.current_stream()
+        h = C.c_void_p()
+        rc = self._L.dgsem_create(C.byref(cfg.to_c()), C.c_void_p(self.stream.cuda_stream), C.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = self._L.dgsem_last_error(h).decode() if h.value else "create failed"
+            if h.value:
+                self._L.dgsem_destroy(h)
+                self._h = C.c_void_p()
+            raise _abi.DgsemError(f"dgsem_create: {msg} ({rc})")
+        E, nn, nv = C.c_int64(), C.c_int(), C.c_int()
+        self._check(self._L.dgsem_sizes(h, C.byref(E), C.byref(nn), C.byref(nv)))
+        self.n_elements, self.ndofs, self.nvars = E.value, nn.value, nv.value
+        self.shape = (self.n_elements, self.nvars, self.ndofs)
+        self.ref_shape = (self.n_elements, self.ndofs, self.nvars)
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.dgsem_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise _abi.DgsemError(f"{self._L.dgsem_last_error(self._h).decode()} ({rc})")
+
+    # -- reference-interface methods ---------------------------------------
+    def get_ndofs(self) -> int:
+        return self.ndofs
+
+    def set_indicator_parameters(self, alpha_max: float, alpha_min: float, alpha_smooth: bool) -> None:
+        self._check(self._L.dgsem_set_indicator_parameters(self._h, alpha_max, alpha_min, int(alpha_smooth)))
+
+    def calc_rhs(self, u, du, t: float = 0.0) -> None:
+        self._check(self._L.dgsem_rhs(self._h, _ptr(u), _ptr(du), float(t)))
+
+    # -- extras --------------------------------------------------------------
+    def empty(self):
+        return torch.empty(self.shape, dtype=torch.float64, device="cuda")
+
+    def node_coordinates(self) -> np.ndarray:
+        out = np.zeros((self.n_elements, self.ndofs, self.cfg.dim))
+        self._check(self._L.dgsem_node_coordinates(self._h, out.ctypes.data))
+        return out
+
+    def contravariant_vectors(self) -> np.ndarray:
+        out = np.zeros((self.n_elements, self.ndofs, self.cfg.dim, self.cfg.dim))
+        self._check(self._L.dgsem_contravariant_vectors(self._h, out.ctypes.data))
+        return out
+
+    def inverse_jacobian(self) -> np.ndarray:
+        out = np.zeros((self.n_elements, self.ndofs))
+        self._check(self._L.dgsem_inverse_jacobian(self._h, out.ctypes.data))
+        return out
+
+    def from_reference_layout(self, u_ref):
+        """u(e, dof, var) (numpy or tensor) -> internal [e][var][dof] on the GPU."""
+        src = torch.as_tensor(np.ascontiguousarray(u_ref) if isinstance(u_ref, np.ndarray) else u_ref,
+                              dtype=torch.float64).to("cuda").contiguous()
+        dst = self.empty()
+        self._check(self._L.dgsem_pack_state(self._h, _ptr(src), _ptr(dst)))
+        return dst
+
+    def to_reference_layout(self, u) -> np.ndarray:
+        dst = torch.empty(self.ref_shape, dtype=torch.float64, device="cuda")
+        self._check(self._L.dgsem_unpack_state(self._h, _ptr(u), _ptr(dst)))
+        self.synchronize()
+        return dst.cpu().numpy()
+
+    def alpha(self):
+        a = torch.empty(self.n_elements, dtype=torch.float64, device="cuda")
+        self._check(self._L.dgsem_get_alpha(self._h, _ptr(a)))
+        return a
+
+    def ssprk3_stage(self, stage: int, dt: float, u_base, u_k, u_out) -> None:
+        base = _ptr(u_base) if u_base is not None else None
+        self._check(self._L.dgsem_ssprk3_stage(self._h, stage, float(dt), base, _ptr(u_k), _ptr(u_out)))
+
+    def ssprk3_steps(self, u, dt: float, nsteps: int, tmp1, tmp2) -> None:
+        self._check(self._L.dgsem_ssprk3_steps(self._h, _ptr(u), float(dt), int(nsteps), _ptr(tmp1), _ptr(tmp2)))
+
+    def synchronize(self) -> None:
+        self._check(self._L.dgsem_synchronize(self._h))
+
+
+class SSPRK3:
+    """Three-stage SSP Runge-Kutta in Shu-Osher form on a solver's state,
+    SSPRK3.hpp:145-183; each stage is one fused kernel."""
+
+    def __init__(self, solver: StructuredSolver, final_time: float = 0.0):
+        self.solver = solver
+        self.tmp1 = solver.empty()
+        self.tmp2 = solver.empty()
+        self.time = 0.0
+        self.final_time = final_time
+        self.iter = 0
+
+    def step(self, u, dt: float) -> None:
+        s = self.solver
+        s.ssprk3_stage(1, dt, None, u, self.tmp1)
+        s.ssprk3_stage(2, dt, u, self.tmp1, self.tmp2)
+        s.ssprk3_stage(3, dt, u, self.tmp2, u)
+
+    def solve(self, u, dt: float) -> None:
+        """TimeIntegrator::solve (integrator_base.hpp:21-40): fixed dt while t < t_final."""
+        while self.time < self.final_time:
+            self.step(u, dt)
+            self.time += dt
+            self.iter += 1

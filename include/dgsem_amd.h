@@ -1,0 +1,156 @@
+/*
+ * dgsem_amd.h -- C ABI of the MI355X DGSEM right-hand-side library.
+ *
+ * The library replaces the reference's StructuredSolver::calc_rhs and the
+ * SSPRK3 stage kernels (RetrO598/DGSEM src/solver/structured_solver.hpp,
+ * src/time_integrator/SSPRK3.hpp) with hand-written HIP kernels for gfx950.
+ * All state pointers are DEVICE pointers to double arrays in the library's
+ * internal layout u[element][var][dof] (element index (ie*ny + je)*nz + ke,
+ * dof = (k*(N+1) + j)*(N+1) + i).  dgsem_pack_state / dgsem_unpack_state
+ * convert from / to the reference's LayoutRight u(ielem,jelem[,kelem],dof,var).
+ *
+ * Every function returns 0 on success and a negative DGSEM_E_* code on
+ * failure; dgsem_last_error() returns the message.  No C++ exception crosses
+ * the ABI.  All work is enqueued on the handle's stream; calls are
+ * asynchronous with respect to the host.  One handle must not be used from
+ * two host threads at once.
+ */
+#ifndef DGSEM_AMD_H
+#define DGSEM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DGSEM_ABI_VERSION 1
+
+/* error codes */
+#define DGSEM_OK 0
+#define DGSEM_E_INVALID (-1)     /* bad argument or configuration */
+#define DGSEM_E_UNSUPPORTED (-2) /* configuration not built */
+#define DGSEM_E_HIP (-3)         /* HIP runtime error */
+#define DGSEM_E_ALLOC (-4)       /* device allocation failed */
+
+/* equations (src/equations/compressible_euler{2D,3D}.hpp,
+ * compressible_navier_stokes{2D,3D}.hpp) */
+#define DGSEM_EULER 0
+#define DGSEM_NAVIER_STOKES 1
+
+/* volume integral policies (src/space_integral/volume_flux.hpp:686-879) */
+#define DGSEM_VOLUME_SPLIT_FORM 0       /* VolumeIntegralSplitForm */
+#define DGSEM_VOLUME_SHOCK_CAPTURING 1  /* VolumeIntegralShockCapturingHG */
+
+/* two-point / numerical fluxes (src/base/numerical_flux.hpp) */
+#define DGSEM_FLUX_CHANDRASHEKAR 0 /* volume flux: ChandrashekarFlux */
+#define DGSEM_FLUX_LLF 0           /* surface / FV flux: LaxFriedrichsFlux */
+#define DGSEM_FLUX_HLLC 1          /* surface / FV flux: HLLC with a normal */
+
+/* geometry representation on the device */
+#define DGSEM_GEOMETRY_NODAL 0  /* per-node metric terms exactly as the
+                                   reference computes them (curl form) */
+#define DGSEM_GEOMETRY_AFFINE 1 /* exact per-element metric terms of the
+                                   linear mapping; 10 doubles per element */
+
+/* boundary conditions (src/boundary_condition) */
+#define DGSEM_BC_PERIODIC 0
+
+/* Solver configuration: the reference's template arguments
+ * StructuredSolver<Equations, Basis, VolumeFlux, SurfaceFlux, Mesh,
+ * BoundarySet> (structured_solver.hpp:19-58) plus the runtime parameters of
+ * the equation, the mesh and set_indicator_parameters (:118-123). */
+typedef struct dgsem_config {
+  int dim;                 /* 2 or 3 */
+  int polydeg;             /* N, 1..7 (LobattoLegendreBasis<T, N>) */
+  int cells[3];            /* StructuredMesh n_cells */
+  double domain_left[3];   /* StructuredMesh domain_left */
+  double domain_right[3];  /* StructuredMesh domain_right */
+  int boundary[3];         /* DGSEM_BC_* per direction (both faces) */
+  int equation;            /* DGSEM_EULER or DGSEM_NAVIER_STOKES */
+  double gamma;            /* default 1.4 */
+  double mu;               /* dynamic viscosity (Navier-Stokes) */
+  double prandtl;          /* Prandtl number (Navier-Stokes) */
+  int volume_integral;     /* DGSEM_VOLUME_* */
+  int volume_flux;         /* DGSEM_FLUX_CHANDRASHEKAR */
+  int surface_flux;        /* DGSEM_FLUX_LLF or DGSEM_FLUX_HLLC */
+  int fv_flux;             /* subcell FV flux for shock capturing */
+  double alpha_max;        /* set_indicator_parameters: default 0.5 */
+  double alpha_min;        /* default 0.001 */
+  int alpha_smooth;        /* stored, unused (as in the reference) */
+  double positivity_floor; /* <= 0 disables; applied after stage updates */
+  int geometry;            /* DGSEM_GEOMETRY_* */
+} dgsem_config;
+
+typedef struct dgsem_handle dgsem_handle;
+
+/* Fills cfg with the reference's defaults for dim and N (periodic box
+ * [-1,1]^dim, 1 cell, Euler, gamma 1.4, split form, LLF, nodal geometry). */
+void dgsem_config_default(dgsem_config* cfg, int dim, int polydeg);
+
+/* Builds basis, geometry and device buffers; replaces the StructuredSolver
+ * constructor (structured_solver.hpp:45-58) with StructuredElementInitializer
+ * (data_container.hpp:735-768) and LobattoLegendreBasis::initialize
+ * (basis.hpp:69-153).  stream: a hipStream_t, or NULL for the null stream. */
+int dgsem_create(const dgsem_config* cfg, void* stream, dgsem_handle** out);
+int dgsem_destroy(dgsem_handle* h);
+const char* dgsem_last_error(const dgsem_handle* h);
+int dgsem_abi_version(void);
+
+/* Number of elements, nodes per element ((N+1)^dim) and conserved variables;
+ * StructuredSolver::get_ndofs (structured_solver.hpp:125). */
+int dgsem_sizes(const dgsem_handle* h, int64_t* n_elements, int* ndofs, int* nvars);
+
+/* set_indicator_parameters(alpha_max, alpha_min, alpha_smooth),
+ * structured_solver.hpp:118-123. */
+int dgsem_set_indicator_parameters(dgsem_handle* h, double alpha_max, double alpha_min,
+                                   int alpha_smooth);
+
+/* Host copies of the geometry in the reference's layouts
+ * (StructuredElementContainer, data_container.hpp:12-60):
+ * coordinates [E][nn][dim], contravariant vectors [E][nn][dim][dim],
+ * inverse Jacobian [E][nn]. */
+int dgsem_node_coordinates(const dgsem_handle* h, double* host_out);
+int dgsem_contravariant_vectors(const dgsem_handle* h, double* host_out);
+int dgsem_inverse_jacobian(const dgsem_handle* h, double* host_out);
+
+/* Layout conversion on the device between the reference's LayoutRight
+ * u(e, dof, var) and the internal u[e][var][dof]. */
+int dgsem_pack_state(dgsem_handle* h, const double* ref_layout, double* internal);
+int dgsem_unpack_state(dgsem_handle* h, const double* internal, double* ref_layout);
+
+/* du = L(u): StructuredSolver::calc_rhs(sol, t), structured_solver.hpp:463-480
+ * (volume integral, interface flux, boundary conditions, parabolic terms,
+ * surface integral, Jacobian).  u and du are device arrays of
+ * n_elements*nvars*ndofs doubles, internal layout.  t is accepted for
+ * interface parity; periodic boundaries do not use it. */
+int dgsem_rhs(dgsem_handle* h, const double* u, double* du, double t);
+
+/* Blending factor per element from the last shock-capturing RHS or stage
+ * (VolumeIntegralShockCapturingHG::calc_alpha, volume_flux.hpp:785-788);
+ * device array of n_elements doubles. */
+int dgsem_get_alpha(const dgsem_handle* h, double* alpha_out);
+
+/* One fused SSPRK3 stage: RHS of u_k followed by the stage combination of
+ * parallel_ma2 / parallel_ma3 (SSPRK3.hpp:12-143) as SSPRK3::step uses them
+ * (SSPRK3.hpp:159-181):
+ *   stage 1: u_out = u_k + dt L(u_k)                      (u_base unused)
+ *   stage 2: u_out = 3/4 u_base + 1/4 u_k + (dt/4) L(u_k)
+ *   stage 3: u_out = 1/3 u_base + 2/3 u_k + (2 dt/3) L(u_k)
+ * u_out may alias u_base (stage 3 writes u^{n+1} over u^n) but never u_k. */
+int dgsem_ssprk3_stage(dgsem_handle* h, int stage, double dt, const double* u_base,
+                       const double* u_k, double* u_out);
+
+/* nsteps SSPRK3 steps of fixed dt on u (SSPRK3::step, SSPRK3.hpp:159-181,
+ * inside TimeIntegrator::solve's loop, integrator_base.hpp:21-40).  tmp1 and
+ * tmp2 are caller-provided device arrays of the state's size. */
+int dgsem_ssprk3_steps(dgsem_handle* h, double* u, double dt, int nsteps, double* tmp1,
+                       double* tmp2);
+
+/* Waits for all work on the handle's stream. */
+int dgsem_synchronize(dgsem_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,51 @@
+"""The C-ABI library loads and exports every entry point include/dgsem_amd.h
+declares (no compute calls: this container has no GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from dgsem_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "dgsem_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dgsem_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_expected_entry_points():
+    names = declared_functions()
+    assert set(names) == set(_abi.EXPORTED), names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _abi.load()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+        assert ctypes.cast(getattr(lib, name), ctypes.c_void_p).value
+
+
+def test_abi_version_and_defaults_without_gpu():
+    lib = _abi.load()
+    assert lib.dgsem_abi_version() == 1
+    c = _abi.Config()
+    lib.dgsem_config_default(ctypes.byref(c), 3, 3)
+    assert (c.dim, c.polydeg, c.gamma, c.alpha_max, c.alpha_min) == (3, 3, 1.4, 0.5, 0.001)
+    assert c.mu == 6.25e-4 and c.prandtl == 0.72
+    lib.dgsem_config_default(ctypes.byref(c), 2, 4)
+    assert c.mu == 5.0e-3 and c.prandtl == 0.73
+
+
+def test_invalid_configuration_is_rejected_with_a_message():
+    lib = _abi.load()
+    c = _abi.Config()
+    lib.dgsem_config_default(ctypes.byref(c), 4, 3)  # dim 4 is invalid
+    h = ctypes.c_void_p()
+    rc = lib.dgsem_create(ctypes.byref(c), None, ctypes.byref(h))
+    assert rc == -1
+    assert b"dim" in lib.dgsem_last_error(h)
+    lib.dgsem_destroy(h)
