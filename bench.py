@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the DGSEM right-hand side on one MI355X.
+
+Workload: 3D Euler Taylor-Green vortex on [0, 2pi]^3, 64^3 elements, N = 3
+(16.8 M nodes x 5 conserved variables), split-form Chandrashekar flux
+differencing, LLF surface flux, SSPRK3 -- the forward-throughput case the
+metric is quoted on.  One "step" is one SSPRK3 step (three fused RHS + stage
+kernels).  The state is resident in HBM before the timed region.
+
+Prints ONE JSON line.  value = FP64 DOF-updates/s per SSPRK3 stage
+= elements * (N+1)^3 / (step time / 3).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import re
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector / matrix peak
+HBM_PEAK_GBS = 8000.0
+# Algorithmic FLOP convention of the volume kernel (per pair / per node), 3D:
+# a Chandrashekar flux with its averaged normal (85) plus the two-sided
+# accumulation (22); 15 per node for the hoisted primitives.
+C_PAIR_3D, C_NODE_3D = 107, 15
+
+
+def tgv_state(coords: np.ndarray, gamma: float = 1.4) -> np.ndarray:
+    """Taylor-Green vortex (Mach 0.1) in conservative variables, reference
+    layout u[e][dof][var]."""
+    x, y, z = coords[..., 0], coords[..., 1], coords[..., 2]
+    rho = np.ones_like(x)
+    v1 = np.sin(x) * np.cos(y) * np.cos(z)
+    v2 = -np.cos(x) * np.sin(y) * np.cos(z)
+    p = 100.0 / gamma + (1.0 / 16.0) * (np.cos(2 * x) * np.cos(2 * z) + 2 * np.cos(2 * y)
+                                       + 2 * np.cos(2 * x) + np.cos(2 * y) * np.cos(2 * z))
+    E = p / (gamma - 1.0) + 0.5 * rho * (v1 * v1 + v2 * v2)
+    return np.stack([rho, rho * v1, rho * v2, np.zeros_like(x), E], axis=-1)
+
+
+def volume_flops(E: int, N: int) -> float:
+    n = N + 1
+    pairs = 3 * n * n * N * (N + 1) // 2
+    return float(E) * (pairs * C_PAIR_3D + n ** 3 * C_NODE_3D)
+
+
+def read_traffic(path: str):
+    """HBM bytes per dispatch of the fused kernel from a committed rocprofv3
+    --pmc summary (FETCH_SIZE / WRITE_SIZE in KiB, already scaled)."""
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    f = re.search(r"fetch_bytes_per_dispatch:\s*([0-9.eE+]+)", text)
+    w = re.search(r"write_bytes_per_dispatch:\s*([0-9.eE+]+)", text)
+    if not (f and w):
+        return None
+    return float(f.group(1)) + float(w.group(1))
+
+
+def cpu_baseline(N: int, cells: int, steps: int) -> dict:
+    """The CPU restatement of the reference (OpenMP over elements, one thread
+    per host core of this box's share) on a bounded sample of the workload."""
+    from oracle import oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    c = O.Case(3, N, (cells,) * 3, [0, 0, 0], [2 * math.pi] * 3, threads=threads)
+    u = c.init_tgv()
+    dt = 1e-4
+    u = c.ssprk3(u, dt, 1)  # warm-up step
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        u = c.ssprk3(u, dt, 1)
+        times.append(time.perf_counter() - t0)
+    t_step = statistics.median(times)
+    dof = c.E * c.ndofs
+    return {"value": dof / (t_step / 3.0), "unit": "DOF-updates/s per SSPRK3 stage",
+            "cores": threads, "kind": "port",
+            "sample": f"TGV N={N} on {cells}^3 elements ({dof} nodes), median of {steps} "
+                      f"SSPRK3 steps after 1 warm-up, oracle/ restatement, OpenMP {threads} threads",
+            "ms_per_step": t_step * 1e3}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cells", type=int, default=64)
+    ap.add_argument("--polydeg", type=int, default=3)
+    ap.add_argument("--geometry", choices=["nodal", "affine"], default="nodal")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-cells", type=int, default=64)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--profile-copy-mib", type=int, default=0,
+                    help="after the run, launch an elementwise copy kernel of this size "
+                         "(calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the RHS would shard by element blocks with a face-trace halo exchange; "
+              "see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import dgsem_amd as dg
+
+    N, cells = args.polydeg, args.cells
+    geometry = dg.GEOMETRY_NODAL if args.geometry == "nodal" else dg.GEOMETRY_AFFINE
+    cfg = dg.SolverConfig(dim=3, polydeg=N, cells=(cells,) * 3, domain_left=(0, 0, 0),
+                          domain_right=(2 * math.pi,) * 3, geometry=geometry)
+    solver = dg.StructuredSolver(cfg)
+    u = solver.from_reference_layout(tgv_state(solver.node_coordinates()))
+    rk = dg.SSPRK3(solver)
+    E, nn = solver.n_elements, solver.ndofs
+    # dt = CFL dx / (3 (2N+1) lambda_max), CFL 0.5, lambda_max = max |v| + c
+    lam = 1.0 + math.sqrt(1.4 * (100.0 / 1.4 + 0.25) / 1.0)
+    dt = 0.5 * (2 * math.pi / cells) / (3 * (2 * N + 1) * lam)
+
+    for _ in range(args.warmup):
+        rk.step(u, dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rk.step(u, dt)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    if not bool(torch.isfinite(u).all()):
+        print("non-finite state after the timed steps", file=sys.stderr)
+        return 1
+    ms_per_step = wall / args.steps * 1e3
+    value = E * nn / (ms_per_step / 3.0 * 1e-3)
+
+    # Dominant kernel: the fused RHS + stage kernel.  Average launch duration
+    # from HIP events on the stream it runs on (torch's current stream).
+    reps = max(6, 3 * min(args.steps, 10))
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for i in range(reps // 3):
+        rk.step(u, dt)
+    ev1.record()
+    torch.cuda.synchronize()
+    t_kernel_ms = ev0.elapsed_time(ev1) / (3 * (reps // 3))
+    flops = volume_flops(E, N)
+    achieved = flops / (t_kernel_ms * 1e-3) / 1e12
+    traffic = read_traffic(os.path.join(ROOT, "profiles", f"rhs_stage_tgv{cells}_n{N}_{args.geometry}_pmc.txt"))
+
+    out = {
+        "metric": "FP64 DOF-updates/s per SSPRK3 stage",
+        "value": value,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Taylor-Green vortex initial state)",
+        "config": {"workload": f"tgv3d_euler_{cells}^3_N{N}_splitform_llf_ssprk3",
+                   "elements": E, "polydeg": N, "nodes": E * nn, "nvars": 5,
+                   "geometry": args.geometry, "dt": dt},
+        "dof_var_updates_per_s": value * 5,
+        "adjoint_forward_step_ratio": None,
+        "roofline": {"bound": "mfma", "kernel": "rhs_kernel (fused volume+surface+stage)",
+                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "flop_per_launch": flops, "kernel_ms": t_kernel_ms},
+    }
+    if args.profile_copy_mib > 0:
+        a = torch.ones(args.profile_copy_mib * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = a.numel() * 8
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(N, args.cpu_cells, args.cpu_steps)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

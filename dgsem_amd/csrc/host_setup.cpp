@@ -234,6 +234,7 @@ Geometry make_geometry(const BasisTables& b, int dim, const int* cells,
   // Node coordinates through LinearMapping, data_container.hpp:193-221/371-414.
   double delta[3];
   for (int k = 0; k < d; ++k) delta[k] = 2.0 / (double)cells[k];
+#pragma omp parallel for schedule(static)
   for (int ie = 0; ie < g.cells[0]; ++ie)
     for (int je = 0; je < g.cells[1]; ++je)
       for (int ke = 0; ke < g.cells[2]; ++ke) {
@@ -247,6 +248,7 @@ Geometry make_geometry(const BasisTables& b, int dim, const int* cells,
           }
       }
   // Covariant Jacobian by the derivative matrix, :223-258/416-482.
+#pragma omp parallel for schedule(static)
   for (int64_t e = 0; e < E; ++e)
     for (int dof = 0; dof < nn; ++dof)
       for (int r = 0; r < d; ++r) {
@@ -261,6 +263,7 @@ Geometry make_geometry(const BasisTables& b, int dim, const int* cells,
         for (int c = 0; c < d; ++c) JM(e, dof, r, c) = acc[c];
       }
   // Contravariant vectors: 2D adjugate (:260-291), 3D curl form (:484-659).
+#pragma omp parallel for schedule(static)
   for (int64_t e = 0; e < E; ++e) {
     if (d == 2) {
       for (int dof = 0; dof < nn; ++dof) {
@@ -301,6 +304,7 @@ Geometry make_geometry(const BasisTables& b, int dim, const int* cells,
     }
   }
   // Inverse Jacobian determinant, :293-312/661-693.
+#pragma omp parallel for schedule(static)
   for (int64_t e = 0; e < E; ++e)
     for (int dof = 0; dof < nn; ++dof) {
       double det;
@@ -331,6 +335,7 @@ Geometry make_geometry(const BasisTables& b, int dim, const int* cells,
   // n_f = n_{f-1} + w_f sum_m D(f,m) Ja(m), global_def.hpp:175-251/299-439.
   const int FN = d == 2 ? n : n * n;
   g.subcell_normals.assign(E * d * FN * (n - 1) * d, 0.0);
+#pragma omp parallel for schedule(static)
   for (int64_t e = 0; e < E; ++e)
     for (int r = 0; r < d; ++r)
       for (int line = 0; line < FN; ++line) {

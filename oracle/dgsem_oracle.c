@@ -562,6 +562,7 @@ static void build_geometry(orc_case* c, const double* L, const double* R,
 #define X(e, dof, comp) c->coords[((e) * (int64_t)nn + (dof)) * d + (comp)]
 #define JM(e, dof, dir, comp) c->jac[(((e) * (int64_t)nn + (dof)) * d + (dir)) * d + (comp)]
   /* calc_jacobian_matrix: :223-258 (2D), :416-482 (3D) */
+#pragma omp parallel for schedule(static)
   for (int64_t e = 0; e < E; ++e)
     for (int dof = 0; dof < nn; ++dof) {
       const int nd[3] = {dof % n, (dof / n) % n, dof / (n * n)};
@@ -578,6 +579,7 @@ static void build_geometry(orc_case* c, const double* L, const double* R,
       }
     }
   /* calc_contravariant_vectors: :260-291 (2D adjugate), :484-659 (3D curl) */
+#pragma omp parallel for schedule(static)
   for (int64_t e = 0; e < E; ++e) {
     if (d == 2) {
       for (int dof = 0; dof < nn; ++dof) {
@@ -633,6 +635,7 @@ static void build_geometry(orc_case* c, const double* L, const double* R,
     }
   }
   /* calc_inverse_jacobian: :293-312 (2D), :661-693 (3D) */
+#pragma omp parallel for schedule(static)
   for (int64_t e = 0; e < E; ++e)
     for (int dof = 0; dof < nn; ++dof) {
       if (d == 2) {
@@ -679,6 +682,7 @@ static void build_subcell_normals(orc_case* c) {
   c->sn1 = (double*)calloc((size_t)(E * per), sizeof(double));
   c->sn2 = (double*)calloc((size_t)(E * per), sizeof(double));
   c->sn3 = d == 3 ? (double*)calloc((size_t)(E * per), sizeof(double)) : NULL;
+#pragma omp parallel for schedule(static)
   for (int64_t e = 0; e < E; ++e) {
     for (int dir = 0; dir < d; ++dir) {
       double* sn = dir == 0 ? c->sn1 : (dir == 1 ? c->sn2 : c->sn3);
