@@ -47,7 +47,7 @@ def load(path: str | None = None):
     global _lib
     if _lib is not None:
         return _lib
-    path = path or LIB_PATH
+    path = path or os.environ.get("DGSEM_AMD_LIBRARY") or LIB_PATH
     if not os.path.exists(path):
         raise DgsemError(f"{path} not built; run __graft_entry__.build() or dgsem_amd/build.py")
     L = C.CDLL(path)

@@ -20,11 +20,14 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC",
 HOST_FLAGS = ["-std=c++20", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-Wall",
               "-fopenmp"]
 
+# (source, compiler, object name, extra defines)
+INSTANCES = [(2, 1), (2, 2), (2, 3), (2, 4), (2, 5), (2, 7), (3, 2), (3, 3), (3, 4), (3, 5), (3, 7)]
 SOURCES = [
-    ("host_setup.cpp", "g++"),
-    ("rhs_kernels.hip", "hipcc"),
-    ("abi.hip", "hipcc"),
-]
+    ("host_setup.cpp", "g++", "host_setup", ()),
+    ("rhs_dispatch.hip", "hipcc", "rhs_dispatch", ()),
+    ("abi.hip", "hipcc", "abi", ()),
+] + [("rhs_inst.hip", "hipcc", f"rhs_inst_{d}d_n{n}", (f"DGSEM_INST_DIM={d}", f"DGSEM_INST_N={n}"))
+     for d, n in INSTANCES]
 
 
 def _deps_newer(src: str, obj: str) -> bool:
@@ -38,33 +41,59 @@ def _deps_newer(src: str, obj: str) -> bool:
     return os.path.getmtime(inc) > t
 
 
-def _compile(src: str, tool: str, force: bool) -> str:
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    if not force and not _deps_newer(src, obj):
+def build(force: bool = True, verbose: bool = False, defines=(), out: str | None = None,
+          build_dir: str | None = None) -> str:
+    """Compiles and links the library.  defines: extra -D flags for the HIP
+    sources (tuning knobs such as DGSEM_WAVES_3D); out/build_dir: where the
+    variant goes (defaults: the in-tree library)."""
+    out = out or LIB
+    bdir = build_dir or BUILD
+    os.makedirs(bdir, exist_ok=True)
+
+    def one(src):
+        name, tool, oname, extra = src
+        obj = os.path.join(bdir, oname + ".o")
+        if not force and not _deps_newer(name, obj):
+            return obj
+        path = os.path.join(CSRC, name)
+        if tool == "g++":
+            cmd = ["g++", *HOST_FLAGS, "-c", path, "-o", obj]
+        else:
+            cmd = [HIPCC, *HIP_FLAGS, *[f"-D{d}" for d in (*defines, *extra)], "-c", path, "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stderr}")
         return obj
-    path = os.path.join(CSRC, src)
-    if tool == "g++":
-        cmd = ["g++", *HOST_FLAGS, "-c", path, "-o", obj]
-    else:
-        cmd = [HIPCC, *HIP_FLAGS, "-c", path, "-o", obj]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stderr}")
-    return obj
 
-
-def build(force: bool = True, verbose: bool = False) -> str:
-    os.makedirs(BUILD, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        objs = list(ex.map(lambda s: _compile(s[0], s[1], force), SOURCES))
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", *objs, "-lgomp"]
+    workers = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        objs = list(ex.map(one, SOURCES))
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp", *objs, "-lgomp"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stderr}")
-    os.replace(LIB + ".tmp", LIB)
+    os.replace(out + ".tmp", out)
+    if out == LIB:
+        build_examples()
     if verbose:
-        print("built", LIB)
-    return LIB
+        print("built", out)
+    return out
+
+
+EXAMPLES = ["khi_shock_capturing"]
+
+
+def build_examples() -> None:
+    """C++ programs written against include/dgsem_hip.hpp (the reference-style
+    template interface), linked to the in-tree library."""
+    for name in EXAMPLES:
+        src = os.path.join(HERE, "examples", name + ".cpp")
+        exe = os.path.join(HERE, "examples", name)
+        cmd = [HIPCC, "-std=c++20", "-O2", "-o", exe, src, f"-L{HERE}", "-ldgsem_amd",
+               "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"example build failed: {' '.join(cmd)}\n{r.stderr}")
 
 
 if __name__ == "__main__":

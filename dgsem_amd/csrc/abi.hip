@@ -35,6 +35,8 @@ struct dgsem_handle {
   int* d_nbr = nullptr;
   double* d_Vinv = nullptr;
   double* d_alpha = nullptr;
+  double* d_grad = nullptr;  // Navier-Stokes gradient [E][dim][var][dof]
+  double* d_visc = nullptr;  // Navier-Stokes viscous RHS part [E][var][dof]
   dgsem::KParams base{};
   std::string err;
 };
@@ -75,7 +77,9 @@ void release(dgsem_handle* h) {
   hipFree(h->d_nbr);
   hipFree(h->d_Vinv);
   hipFree(h->d_alpha);
-  h->d_Ja = h->d_invJ = h->d_sn = h->d_Vinv = h->d_alpha = nullptr;
+  hipFree(h->d_grad);
+  hipFree(h->d_visc);
+  h->d_Ja = h->d_invJ = h->d_sn = h->d_Vinv = h->d_alpha = h->d_grad = h->d_visc = nullptr;
   h->d_nbr = nullptr;
 }
 
@@ -164,8 +168,17 @@ void fill_params(dgsem_handle* h) {
   p.lhatN = b.lhat_right[b.n - 1];
   for (int i = 0; i < b.n; ++i) {
     p.inv_w[i] = b.inv_weights[i];
-    for (int j = 0; j < b.n; ++j) p.Ds[i][j] = b.Dsplit[i * b.n + j];
+    for (int j = 0; j < b.n; ++j) {
+      p.Ds[i][j] = b.Dsplit[i * b.n + j];
+      p.Dhat[i][j] = b.Dhat[i * b.n + j];
+    }
   }
+  // Navier-Stokes: kappa = gamma / ((gamma - 1) Pr), compressible_navier_stokes3D.hpp:19-22
+  p.mu = c.mu;
+  p.kappa = c.gamma / ((c.gamma - 1.0) * c.prandtl);
+  p.grad = h->d_grad;
+  p.grad_out = h->d_grad;
+  p.visc_out = h->d_visc;
 }
 
 int check_config(dgsem_handle* h, const dgsem_config& c) {
@@ -180,8 +193,10 @@ int check_config(dgsem_handle* h, const dgsem_config& c) {
   }
   const int64_t E = (int64_t)c.cells[0] * c.cells[1] * (c.dim == 3 ? c.cells[2] : 1);
   if (E > std::numeric_limits<int>::max() / 2) return fail(h, DGSEM_E_INVALID, "mesh too large");
-  if (c.equation != DGSEM_EULER)
-    return fail(h, DGSEM_E_UNSUPPORTED, "Navier-Stokes terms are not built in this library");
+  if (c.equation != DGSEM_EULER && c.equation != DGSEM_NAVIER_STOKES)
+    return fail(h, DGSEM_E_INVALID, "unknown equation");
+  if (c.equation == DGSEM_NAVIER_STOKES && !(c.mu >= 0.0 && c.prandtl > 0.0))
+    return fail(h, DGSEM_E_INVALID, "Navier-Stokes needs mu >= 0 and Pr > 0");
   if (c.volume_integral != DGSEM_VOLUME_SPLIT_FORM && c.volume_integral != DGSEM_VOLUME_SHOCK_CAPTURING)
     return fail(h, DGSEM_E_INVALID, "unknown volume integral");
   if (c.volume_flux != DGSEM_FLUX_CHANDRASHEKAR) return fail(h, DGSEM_E_INVALID, "unknown volume flux");
@@ -198,9 +213,18 @@ int check_config(dgsem_handle* h, const dgsem_config& c) {
 
 hipStream_t stream_of(const dgsem_handle* h) { return h->stream; }
 
-int launch(dgsem_handle* h, const dgsem::KParams& p) {
+int launch(dgsem_handle* h, dgsem::KParams p) {
   const bool sc = h->cfg.volume_integral == DGSEM_VOLUME_SHOCK_CAPTURING;
   const bool nodal = h->cfg.geometry == DGSEM_GEOMETRY_NODAL;
+  if (h->cfg.equation == DGSEM_NAVIER_STOKES) {
+    // BR1 gradient and viscous part of the stage input, then the inviscid
+    // kernel adds the viscous part before -1/J
+    hipError_t e = dgsem::launch_parabolic(h->cfg.dim, h->cfg.polydeg, nodal, p, stream_of(h));
+    if (e != hipSuccess) return hip_fail(h, e, "parabolic kernel launch");
+    p.visc = h->d_visc;
+  } else {
+    p.visc = nullptr;
+  }
   hipError_t e = dgsem::launch_rhs(h->cfg.dim, h->cfg.polydeg, sc, nodal, p, stream_of(h));
   if (e != hipSuccess) return hip_fail(h, e, "rhs kernel launch");
   return DGSEM_OK;
@@ -276,7 +300,10 @@ int dgsem_create(const dgsem_config* cfg, void* stream, dgsem_handle** out) {
   if ((rc = upload(h, &h->d_Ja, Ja)) || (rc = upload(h, &h->d_invJ, invJ)) ||
       (rc = upload(h, &h->d_sn, sn)) || (rc = upload(h, &h->d_nbr, neighbour_table(h->geo))) ||
       (rc = upload(h, &h->d_Vinv, h->basis.Vinv)) ||
-      (rc = upload(h, &h->d_alpha, std::vector<double>(h->E, 0.0)))) {
+      (rc = upload(h, &h->d_alpha, std::vector<double>(h->E, 0.0))) ||
+      (h->cfg.equation == DGSEM_NAVIER_STOKES &&
+       ((rc = upload(h, &h->d_grad, std::vector<double>(h->E * h->cfg.dim * h->nvars * h->nn, 0.0))) ||
+        (rc = upload(h, &h->d_visc, std::vector<double>(h->E * h->nvars * h->nn, 0.0)))))) {
     release(h);
     *out = h;
     return rc;

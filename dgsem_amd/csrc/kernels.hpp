@@ -26,6 +26,10 @@ struct KParams {
   const double* sn;      // nodal subcell normals [E][dim][FN][N][dim]
   const int* nbr;        // [E][2*dim] (lower, upper) neighbour per direction
   const double* Vinv;    // [n][n] inverse Legendre Vandermonde
+  const double* grad;    // Navier-Stokes: gradient [E][dim][var][dof]
+  double* grad_out;      // written by the gradient kernel
+  const double* visc;    // Navier-Stokes: viscous part of the RHS before -1/J
+  double* visc_out;      // written by the viscous kernel
   int64_t E;
   int mode;
   int surface_flux, fv_flux;
@@ -34,9 +38,21 @@ struct KParams {
   double alpha_max, alpha_min, atol, threshold, sfac;
   double floor_value;
   double lhat0, lhatN;
+  double mu, kappa;  // viscosity and gamma / ((gamma - 1) Pr)
   double Ds[kMaxN1][kMaxN1];
+  double Dhat[kMaxN1][kMaxN1];
   double inv_w[kMaxN1];
 };
+
+// Per-(dim, N) launchers, explicitly specialised in rhs_inst.hip.
+template <int DIM, int N>
+hipError_t launch_rhs_dn(bool shock_capturing, bool nodal, const KParams& p, hipStream_t stream);
+template <int DIM, int N>
+hipError_t launch_parabolic_dn(bool nodal, const KParams& p, hipStream_t stream);
+
+// Gradient and viscous kernels of the Navier-Stokes terms: fill p.grad_out
+// (with p.grad == p.grad_out) and p.visc_out from p.u.
+hipError_t launch_parabolic(int dim, int N, bool nodal, const KParams& p, hipStream_t stream);
 
 // Launches the fused RHS / stage kernel for the configuration; returns
 // hipErrorInvalidValue when (dim, N, shock capturing, nodal) is not built.

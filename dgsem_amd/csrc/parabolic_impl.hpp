@@ -1,0 +1,288 @@
+// BR1 parabolic terms of the compressible Navier-Stokes RHS on the GPU.
+//
+// Two kernels per RHS, both one workgroup per EPW elements with line threads
+// as in the inviscid kernel:
+//   * gradient_kernel: q = (rho, v, T) at the nodes; along each line the weak
+//     derivative D_hat q, the BR1 face value q_hat = (q_L + q_R)/2 lifted
+//     with l_hat; then the physical gradient sum_d Ja^d (.) and 1/J.
+//     Reference: GradientVolumeIntegralFunctor, GradientPhysicalTransform-
+//     Functor, GradientInterfaceFluxFunctor, GradientSurfaceIntegralFunctor,
+//     GradientJacobianProjFunctor (parabolic_gradient.hpp:17-478 3D,
+//     :480-818 2D).
+//   * viscous_kernel: the contravariant viscous flux Ja^d . (-F_v) at the
+//     nodes, its weak D_hat divergence along each line, and the interface
+//     flux, which is the upper ("primary") element's (-F_v).n only
+//     (parabolic_viscous.hpp:318-334 3D, :570-585 2D), lifted with l_hat.
+//     The result (before -1/J) is added by the inviscid kernel.
+// Layouts: gradient [E][dim][var][dof]; viscous part [E][var][dof].
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "physics.hpp"
+#include "rhs_kernel_impl.hpp"
+
+namespace dgsem {
+namespace detail {
+
+// gradient_variables (rho, v, T = p/rho), compressible_navier_stokes3D.hpp:55-70
+template <int DIM>
+__device__ __forceinline__ void grad_vars(double gamma, const double* u, double* q) {
+  const double inv_rho = 1.0 / u[0];
+  double m2 = 0.0;
+  for (int k = 0; k < DIM; ++k) m2 += u[k + 1] * u[k + 1];
+  const double p = (gamma - 1.0) * (u[DIM + 1] - 0.5 * m2 * inv_rho);
+  q[0] = u[0];
+  for (int k = 0; k < DIM; ++k) q[k + 1] = u[k + 1] * inv_rho;
+  q[DIM + 1] = p * inv_rho;
+}
+
+// Newtonian viscous flux in direction dim, compressible_navier_stokes{2D,3D}.hpp
+// viscous_flux.  g[c][var] = d q_var / d x_c.
+template <int DIM>
+__device__ __forceinline__ void viscous_flux_dir(double mu, double kappa, const double* q,
+                                                 const double g[DIM][DIM + 2], int dim, double* f) {
+  if constexpr (DIM == 2) {
+    const double tau11 = (4.0 * g[0][1] - 2.0 * g[1][2]) / 3.0;
+    const double tau22 = (4.0 * g[1][2] - 2.0 * g[0][1]) / 3.0;
+    const double tau12 = g[1][1] + g[0][2];
+    const double t1 = dim == 0 ? tau11 : tau12, t2 = dim == 0 ? tau12 : tau22;
+    f[0] = 0.0;
+    f[1] = mu * t1;
+    f[2] = mu * t2;
+    f[3] = mu * (q[1] * t1 + q[2] * t2 + kappa * g[dim][3]);
+  } else {
+    const double tau11 = (4.0 * g[0][1] - 2.0 * (g[1][2] + g[2][3])) / 3.0;
+    const double tau22 = (4.0 * g[1][2] - 2.0 * (g[0][1] + g[2][3])) / 3.0;
+    const double tau33 = (4.0 * g[2][3] - 2.0 * (g[0][1] + g[1][2])) / 3.0;
+    const double tau12 = g[1][1] + g[0][2];
+    const double tau13 = g[2][1] + g[0][3];
+    const double tau23 = g[2][2] + g[1][3];
+    const double t1 = dim == 0 ? tau11 : (dim == 1 ? tau12 : tau13);
+    const double t2 = dim == 0 ? tau12 : (dim == 1 ? tau22 : tau23);
+    const double t3 = dim == 0 ? tau13 : (dim == 1 ? tau23 : tau33);
+    f[0] = 0.0;
+    f[1] = mu * t1;
+    f[2] = mu * t2;
+    f[3] = mu * t3;
+    f[4] = mu * (q[1] * t1 + q[2] * t2 + q[3] * t3 + kappa * g[dim][4]);
+  }
+}
+
+template <bool NODAL, int DIM, int NN>
+__device__ __forceinline__ double metric(const KParams& p, int64_t e, int r, int c, int dof) {
+  if constexpr (NODAL)
+    return p.Ja[((e * DIM + r) * DIM + c) * NN + dof];
+  else
+    return p.Ja[(e * DIM + r) * DIM + c];
+}
+
+template <bool NODAL, int NN>
+__device__ __forceinline__ double inv_jac(const KParams& p, int64_t e, int dof) {
+  if constexpr (NODAL)
+    return p.invJ[e * NN + dof];
+  else
+    return p.invJ[e];
+}
+
+template <int DIM, int N, int EPW, bool NODAL>
+__global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) gradient_kernel(const KParams p) {
+  using S = Shape<DIM, N>;
+  constexpr int n = S::n, NV = S::NV, NN = S::NN, FN = S::FN, L = S::LINES;
+  constexpr int BLOCK = EPW * L;
+  __shared__ double s_q[EPW * NV * NN];
+  __shared__ double s_g[EPW * DIM * NV * NN];  // per direction: D_hat q + lift
+  const int tid = threadIdx.x;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t e0 = (int64_t)blk * EPW;
+  const int nvalid = (int)((p.E - e0) < EPW ? (p.E - e0) : EPW);
+
+  for (int i = tid; i < EPW * NN; i += BLOCK) {
+    const int sl = i / NN, node = i % NN;
+    if (sl >= nvalid) continue;
+    double s[NV], q[NV];
+    for (int v = 0; v < NV; ++v) s[v] = p.u[((e0 + sl) * NV + v) * NN + node];
+    grad_vars<DIM>(p.gamma, s, q);
+    for (int v = 0; v < NV; ++v) s_q[(sl * NV + v) * NN + node] = q[v];
+  }
+  __syncthreads();
+
+  const int slot = tid / L, rem = tid % L, dir = rem / FN, line = rem % FN;
+  if (slot < nvalid) {
+    const int64_t e = e0 + slot;
+    int dofs[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) dofs[k] = line_dof<DIM, N>(dir, line, k);
+    const double* q = s_q + slot * NV * NN;
+    double acc[n][NV];
+#pragma unroll
+    for (int k = 0; k < n; ++k)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        double r = 0.0;
+#pragma unroll
+        for (int i = 0; i < n; ++i) r += p.Dhat[k][i] * q[v * NN + dofs[i]];
+        acc[k][v] = r;
+      }
+    // BR1 face values q_hat = (q_lower + q_upper) / 2 on both line ends
+#pragma unroll 1
+    for (int side = 0; side < 2; ++side) {
+      const int64_t nb = p.nbr[e * 2 * DIM + 2 * dir + side];
+      const int dof_nb = side == 0 ? dofs[n - 1] : dofs[0];
+      double s[NV], qn[NV];
+      for (int v = 0; v < NV; ++v) s[v] = p.u[(nb * NV + v) * NN + dof_nb];
+      grad_vars<DIM>(p.gamma, s, qn);
+      // the face term uses the element's own normal sign(J) Ja^dir at its node
+      const int dof_own = side == 0 ? dofs[0] : dofs[n - 1];
+      const double sg = inv_jac<NODAL, NN>(p, e, dof_own) >= 0.0 ? 1.0 : -1.0;
+      if (side == 0) {
+        for (int v = 0; v < NV; ++v) acc[0][v] -= sg * p.lhat0 * (0.5 * (qn[v] + q[v * NN + dofs[0]]));
+      } else {
+        for (int v = 0; v < NV; ++v) acc[n - 1][v] += sg * p.lhatN * (0.5 * (q[v * NN + dofs[n - 1]] + qn[v]));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < n; ++k)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) s_g[((slot * DIM + dir) * NV + v) * NN + dofs[k]] = acc[k][v];
+  }
+  __syncthreads();
+  // physical gradient: g_c = (sum_d Ja^d_c G_d) / J
+  for (int i = tid; i < EPW * NN; i += BLOCK) {
+    const int sl = i / NN, node = i % NN;
+    if (sl >= nvalid) continue;
+    const int64_t e = e0 + sl;
+    const double ij = inv_jac<NODAL, NN>(p, e, node);
+    for (int c = 0; c < DIM; ++c) {
+      double ja[DIM];
+      for (int d = 0; d < DIM; ++d) ja[d] = metric<NODAL, DIM, NN>(p, e, d, c, node);
+      for (int v = 0; v < NV; ++v) {
+        double g = 0.0;
+        for (int d = 0; d < DIM; ++d) g += ja[d] * s_g[((sl * DIM + d) * NV + v) * NN + node];
+        p.grad_out[((e * DIM + c) * NV + v) * NN + node] = g * ij;
+      }
+    }
+  }
+}
+
+// Contravariant viscous flux Ja^d . (-F_v) of one node: out[d][v].
+template <int DIM, int NN, bool NODAL>
+__device__ __forceinline__ void contravariant_viscous(const KParams& p, int64_t e, int node,
+                                                      const double* u, int dir_only,
+                                                      double out[DIM][DIM + 2]) {
+  constexpr int NV = DIM + 2;
+  double q[NV], g[DIM][NV], fv[DIM][NV];
+  grad_vars<DIM>(p.gamma, u, q);
+  for (int c = 0; c < DIM; ++c)
+    for (int v = 0; v < NV; ++v) g[c][v] = p.grad[((e * DIM + c) * NV + v) * NN + node];
+  for (int c = 0; c < DIM; ++c) viscous_flux_dir<DIM>(p.mu, p.kappa, q, g, c, fv[c]);
+  for (int d = 0; d < DIM; ++d) {
+    if (dir_only >= 0 && d != dir_only) continue;
+    double ja[DIM];
+    for (int c = 0; c < DIM; ++c) ja[c] = metric<NODAL, DIM, NN>(p, e, d, c, node);
+    for (int v = 0; v < NV; ++v) {
+      double r = 0.0;
+      for (int c = 0; c < DIM; ++c) r += ja[c] * -fv[c][v];
+      out[d][v] = r;
+    }
+  }
+}
+
+template <int DIM, int N, int EPW, bool NODAL>
+__global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) viscous_kernel(const KParams p) {
+  using S = Shape<DIM, N>;
+  constexpr int n = S::n, NV = S::NV, NN = S::NN, FN = S::FN, L = S::LINES;
+  constexpr int BLOCK = EPW * L;
+  __shared__ double s_f[EPW * DIM * NV * NN];  // Ja^d . (-F_v) per node and d
+  __shared__ double s_du[EPW * NV * NN];
+  const int tid = threadIdx.x;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t e0 = (int64_t)blk * EPW;
+  const int nvalid = (int)((p.E - e0) < EPW ? (p.E - e0) : EPW);
+
+  for (int i = tid; i < EPW * NN; i += BLOCK) {
+    const int sl = i / NN, node = i % NN;
+    if (sl >= nvalid) continue;
+    const int64_t e = e0 + sl;
+    double s[NV], out[DIM][NV];
+    for (int v = 0; v < NV; ++v) s[v] = p.u[(e * NV + v) * NN + node];
+    contravariant_viscous<DIM, NN, NODAL>(p, e, node, s, -1, out);
+    for (int d = 0; d < DIM; ++d)
+      for (int v = 0; v < NV; ++v) s_f[((sl * DIM + d) * NV + v) * NN + node] = out[d][v];
+  }
+  __syncthreads();
+
+  const int slot = tid / L, rem = tid % L, dir = rem / FN, line = rem % FN;
+  const bool valid = slot < nvalid;
+  double acc[n][NV];
+  int dofs[n];
+#pragma unroll
+  for (int k = 0; k < n; ++k) dofs[k] = line_dof<DIM, N>(dir, line, k);
+  if (valid) {
+    const int64_t e = e0 + slot;
+    const double* f = s_f + (slot * DIM + dir) * NV * NN;
+#pragma unroll
+    for (int k = 0; k < n; ++k)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        double r = 0.0;
+#pragma unroll
+        for (int i = 0; i < n; ++i) r += p.Dhat[k][i] * f[v * NN + dofs[i]];
+        acc[k][v] = r;
+      }
+    // lower face: this element is primary, (-F_v).(sign(J) Ja^dir) at its node
+    {
+      const double sg = inv_jac<NODAL, NN>(p, e, dofs[0]) >= 0.0 ? 1.0 : -1.0;
+      for (int v = 0; v < NV; ++v) acc[0][v] -= sg * f[v * NN + dofs[0]] * p.lhat0;
+    }
+    // upper face: the upper neighbour is primary
+    {
+      const int64_t nb = p.nbr[e * 2 * DIM + 2 * dir + 1];
+      double s[NV], out[DIM][NV];
+      for (int v = 0; v < NV; ++v) s[v] = p.u[(nb * NV + v) * NN + dofs[0]];
+      contravariant_viscous<DIM, NN, NODAL>(p, nb, dofs[0], s, dir, out);
+      const double sg = inv_jac<NODAL, NN>(p, nb, dofs[0]) >= 0.0 ? 1.0 : -1.0;
+      for (int v = 0; v < NV; ++v) acc[n - 1][v] += sg * out[dir][v] * p.lhatN;
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    if (valid && dir == d) {
+#pragma unroll
+      for (int k = 0; k < n; ++k)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          const int idx = (slot * NV + v) * NN + dofs[k];
+          s_du[idx] = d == 0 ? acc[k][v] : s_du[idx] + acc[k][v];
+        }
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < EPW * NV * NN; i += BLOCK) {
+    if (i / (NV * NN) >= nvalid) continue;
+    p.visc_out[e0 * NV * NN + i] = s_du[i];
+  }
+}
+
+template <int DIM, int N>
+constexpr int parabolic_elems_per_block() {
+  constexpr int e = 192 / Shape<DIM, N>::LINES;
+  return e < 1 ? 1 : e;
+}
+
+template <int DIM, int N, bool NODAL>
+hipError_t launch_parabolic_t(const KParams& p, hipStream_t s) {
+  constexpr int EPW = parabolic_elems_per_block<DIM, N>();
+  constexpr int BLOCK = EPW * Shape<DIM, N>::LINES;
+  const int64_t nb = (p.E + EPW - 1) / EPW;
+  if (nb <= 0) return hipSuccess;
+  hipLaunchKernelGGL((gradient_kernel<DIM, N, EPW, NODAL>), dim3((unsigned)nb), dim3(BLOCK), 0, s, p);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL((viscous_kernel<DIM, N, EPW, NODAL>), dim3((unsigned)nb), dim3(BLOCK), 0, s, p);
+  return hipGetLastError();
+}
+
+}  // namespace detail
+}  // namespace dgsem

@@ -165,11 +165,12 @@ DG_HD void surface_flux(int kind, double gamma, const double* ul, const double* 
     flux_llf<DIM>(gamma, ul, ur, n, f);
 }
 
-// Per-node quantities hoisted out of the two-point flux: rho, v[DIM],
-// beta = rho / (2 p) and the specific kinetic energy |v|^2 / 2.
+// Per-node quantities hoisted out of the two-point and surface fluxes:
+// rho, v[DIM], beta = rho / (2 p), the specific kinetic energy |v|^2 / 2,
+// the pressure and the speed of sound.
 template <int DIM>
 struct NodePrims {
-  double rho, v[DIM], beta, kin;
+  double rho, v[DIM], beta, kin, p, c;
 };
 
 template <int DIM>
@@ -179,14 +180,24 @@ DG_HD NodePrims<DIM> node_prims(double gamma, const double* u) {
   double v2 = 0.0;
   for (int k = 0; k < DIM; ++k) q.v[k] = u[k + 1] / u[0];
   for (int k = 0; k < DIM; ++k) v2 += q.v[k] * q.v[k];
-  const double p = (gamma - 1.0) * (u[DIM + 1] - 0.5 * u[0] * v2);
-  q.beta = 0.5 * u[0] / p;
+  q.p = (gamma - 1.0) * (u[DIM + 1] - 0.5 * u[0] * v2);
+  q.beta = 0.5 * u[0] / q.p;
   q.kin = 0.5 * v2;
+  q.c = sqrt(gamma * q.p / u[0]);
   return q;
 }
 
 // Chandrashekar two-point flux from hoisted node quantities,
 // numerical_flux.hpp:436-500 (2D) and :520-594 (3D).  inv_gm1 = 1/(gamma-1).
+//
+// The two logarithmic means (math_utils.hpp:8-24), the mean pressure
+// 0.5 rho_avg / beta_avg and 1 / beta_mean need five quotients; in the usual
+// case where both means take the series branch (f2 < 1e-4) they come from
+// two reciprocals:  1/(b_rho b_beta) gives f2 of both means, and
+// 1/(poly_rho s_beta) gives rho_mean = s_rho/poly_rho, 1/beta_mean =
+// poly_beta/s_beta and p_mean = s_rho/(2 s_beta), where s = x + y and
+// b = x(x+2y) + y^2.  Results agree with the quotient form to a few ulp.
+// Otherwise the means are evaluated exactly as the reference does.
 template <int DIM>
 DG_HD void flux_chandrashekar(double inv_gm1, const NodePrims<DIM>& a, const NodePrims<DIM>& b,
                               const double* n, double* f) {
@@ -195,14 +206,29 @@ DG_HD void flux_chandrashekar(double inv_gm1, const NodePrims<DIM>& a, const Nod
     vna += a.v[k] * n[k];
     vnb += b.v[k] * n[k];
   }
-  const double rho_avg = 0.5 * (a.rho + b.rho);
-  const double rho_mean = ln_mean(a.rho, b.rho);
-  const double beta_mean = ln_mean(a.beta, b.beta);
-  const double beta_avg = 0.5 * (a.beta + b.beta);
-  const double p_mean = 0.5 * rho_avg / beta_avg;
+  const double sr = a.rho + b.rho, sb = a.beta + b.beta;
+  const double nr = a.rho * (a.rho - 2.0 * b.rho) + b.rho * b.rho;
+  const double dr = a.rho * (a.rho + 2.0 * b.rho) + b.rho * b.rho;
+  const double nb = a.beta * (a.beta - 2.0 * b.beta) + b.beta * b.beta;
+  const double db = a.beta * (a.beta + 2.0 * b.beta) + b.beta * b.beta;
+  const double r1 = 1.0 / (dr * db);
+  const double f2r = nr * (db * r1), f2b = nb * (dr * r1);
+  const double polyr = 2.0 + f2r * (2.0 / 3.0 + f2r * (2.0 / 5.0 + f2r * (2.0 / 7.0)));
+  const double polyb = 2.0 + f2b * (2.0 / 3.0 + f2b * (2.0 / 5.0 + f2b * (2.0 / 7.0)));
+  double rho_mean, inv_beta_mean, p_mean;
+  if (f2r < 1.0e-4 && f2b < 1.0e-4) {
+    const double r2 = 1.0 / (polyr * sb);
+    rho_mean = sr * (sb * r2);
+    inv_beta_mean = polyb * (polyr * r2);
+    p_mean = 0.5 * sr * (polyr * r2);
+  } else {
+    rho_mean = f2r < 1.0e-4 ? sr / polyr : (b.rho - a.rho) / log(b.rho / a.rho);
+    inv_beta_mean = f2b < 1.0e-4 ? polyb / sb : log(b.beta / a.beta) / (b.beta - a.beta);
+    p_mean = 0.5 * sr / sb;
+  }
   const double vsq_avg = a.kin + b.kin;
   const double f1 = rho_mean * 0.5 * (vna + vnb);
-  double fe = f1 * 0.5 * (inv_gm1 / beta_mean - vsq_avg);
+  double fe = f1 * 0.5 * (inv_gm1 * inv_beta_mean - vsq_avg);
   f[0] = f1;
   for (int k = 0; k < DIM; ++k) {
     const double vavg = 0.5 * (a.v[k] + b.v[k]);
@@ -210,6 +236,49 @@ DG_HD void flux_chandrashekar(double inv_gm1, const NodePrims<DIM>& a, const Nod
     fe = fe + f[k + 1] * vavg;
   }
   f[DIM + 1] = fe;
+}
+
+// Primitive quantities of a state read from memory: one reciprocal of rho
+// and one square root.
+template <int DIM>
+DG_HD NodePrims<DIM> face_prims(double gamma, const double* u) {
+  NodePrims<DIM> q;
+  const double ir = 1.0 / u[0];
+  double v2 = 0.0;
+  q.rho = u[0];
+  for (int k = 0; k < DIM; ++k) q.v[k] = u[k + 1] * ir;
+  for (int k = 0; k < DIM; ++k) v2 += q.v[k] * q.v[k];
+  q.p = (gamma - 1.0) * (u[DIM + 1] - 0.5 * u[0] * v2);
+  q.c = sqrt(gamma * q.p * ir);
+  q.beta = 0.0;
+  q.kin = 0.5 * v2;
+  return q;
+}
+
+// Local Lax-Friedrichs flux from conservative states and their primitives:
+// 0.5 (F_L + F_R).n - 0.5 lambda (u_R - u_L), lambda = max|v.n| + max(c)|n|
+// (numerical_flux.hpp:54-68, compressible_euler3D.hpp:121-166).
+template <int DIM>
+DG_HD void flux_llf_prims(const double* ul, const NodePrims<DIM>& ql, const double* ur,
+                          const NodePrims<DIM>& qr, const double* n, double* f) {
+  constexpr int NV = DIM + 2;
+  double vnl = 0.0, vnr = 0.0, nn = 0.0;
+  for (int k = 0; k < DIM; ++k) {
+    vnl += ql.v[k] * n[k];
+    vnr += qr.v[k] * n[k];
+    nn += n[k] * n[k];
+  }
+  const double lam = dmax(fabs(vnl), fabs(vnr)) + dmax(ql.c, qr.c) * sqrt(nn);
+  double fl[NV], fr[NV];
+  fl[0] = ul[0] * vnl;
+  fr[0] = ur[0] * vnr;
+  for (int k = 0; k < DIM; ++k) {
+    fl[k + 1] = ul[k + 1] * vnl + ql.p * n[k];
+    fr[k + 1] = ur[k + 1] * vnr + qr.p * n[k];
+  }
+  fl[NV - 1] = (ul[NV - 1] + ql.p) * vnl;
+  fr[NV - 1] = (ur[NV - 1] + qr.p) * vnr;
+  for (int i = 0; i < NV; ++i) f[i] = 0.5 * (fl[i] + fr[i]) - 0.5 * lam * (ur[i] - ul[i]);
 }
 
 // p * rho with gamma fixed at 1.4, the indicator variable of

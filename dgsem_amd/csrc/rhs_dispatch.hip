@@ -1,0 +1,90 @@
+// Dispatch of the fused RHS kernel over (dim, N) and the layout-conversion
+// kernel.  The kernel body is in rhs_kernel_impl.hpp; its instantiations are
+// compiled per (dim, N) from rhs_inst.hip.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace dgsem {
+namespace {
+
+__global__ void pack_kernel(const double* __restrict__ src, double* __restrict__ dst, int64_t E,
+                            int nn, int nv, int to_internal) {
+  const int64_t total = E * nn * nv;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    // i enumerates the internal layout [e][v][dof]
+    const int64_t e = i / ((int64_t)nn * nv);
+    const int r = (int)(i % ((int64_t)nn * nv));
+    const int v = r / nn, dof = r % nn;
+    const int64_t j = (e * nn + dof) * nv + v;  // reference layout [e][dof][v]
+    if (to_internal)
+      dst[i] = src[j];
+    else
+      dst[j] = src[i];
+  }
+}
+
+}  // namespace
+
+hipError_t launch_rhs(int dim, int N, bool sc, bool nodal, const KParams& p, hipStream_t s) {
+  if (dim == 2) {
+    switch (N) {
+      case 1: return launch_rhs_dn<2, 1>(sc, nodal, p, s);
+      case 2: return launch_rhs_dn<2, 2>(sc, nodal, p, s);
+      case 3: return launch_rhs_dn<2, 3>(sc, nodal, p, s);
+      case 4: return launch_rhs_dn<2, 4>(sc, nodal, p, s);
+      case 5: return launch_rhs_dn<2, 5>(sc, nodal, p, s);
+      case 7: return launch_rhs_dn<2, 7>(sc, nodal, p, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  if (dim == 3) {
+    switch (N) {
+      case 2: return launch_rhs_dn<3, 2>(sc, nodal, p, s);
+      case 3: return launch_rhs_dn<3, 3>(sc, nodal, p, s);
+      case 4: return launch_rhs_dn<3, 4>(sc, nodal, p, s);
+      case 5: return launch_rhs_dn<3, 5>(sc, nodal, p, s);
+      case 7: return launch_rhs_dn<3, 7>(sc, nodal, p, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  return hipErrorInvalidValue;
+}
+hipError_t launch_parabolic(int dim, int N, bool nodal, const KParams& p, hipStream_t s) {
+  if (dim == 2) {
+    switch (N) {
+      case 1: return launch_parabolic_dn<2, 1>(nodal, p, s);
+      case 2: return launch_parabolic_dn<2, 2>(nodal, p, s);
+      case 3: return launch_parabolic_dn<2, 3>(nodal, p, s);
+      case 4: return launch_parabolic_dn<2, 4>(nodal, p, s);
+      case 5: return launch_parabolic_dn<2, 5>(nodal, p, s);
+      case 7: return launch_parabolic_dn<2, 7>(nodal, p, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  if (dim == 3) {
+    switch (N) {
+      case 2: return launch_parabolic_dn<3, 2>(nodal, p, s);
+      case 3: return launch_parabolic_dn<3, 3>(nodal, p, s);
+      case 4: return launch_parabolic_dn<3, 4>(nodal, p, s);
+      case 5: return launch_parabolic_dn<3, 5>(nodal, p, s);
+      case 7: return launch_parabolic_dn<3, 7>(nodal, p, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_pack(const double* src, double* dst, int64_t E, int nn, int nv,
+                       bool to_internal, hipStream_t stream) {
+  const int64_t total = E * nn * nv;
+  if (total <= 0) return hipSuccess;
+  int64_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(pack_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, src, dst, E, nn,
+                     nv, to_internal ? 1 : 0);
+  return hipGetLastError();
+}
+
+}  // namespace dgsem

@@ -20,13 +20,15 @@
 // HGIndicator{2D,3D} (indicator.hpp:313-585), interface_flux
 // (surface_flux.hpp:45-271), surface_integral (surface_integral.hpp:35-154),
 // jacobian_proj (jacobian.hpp:22-105), parallel_ma2/ma3 (SSPRK3.hpp:12-143).
+#pragma once
+
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
 #include "physics.hpp"
 
 namespace dgsem {
-namespace {
+namespace detail {
 
 template <int DIM, int N>
 struct Shape {
@@ -35,7 +37,7 @@ struct Shape {
   static constexpr int NN = DIM == 2 ? n * n : n * n * n;
   static constexpr int FN = DIM == 2 ? n : n * n;  // lines per direction
   static constexpr int LINES = DIM * FN;            // line threads per element
-  static constexpr int NP = DIM + 3;                // rho, v[DIM], beta, kin
+  static constexpr int NP = DIM + 5;                // rho, v[DIM], beta, kin, p, c
 };
 
 // Elements per workgroup: keeps EPW * LINES near 192 threads.
@@ -153,16 +155,44 @@ __device__ void block_alpha(const KParams& p, const double* s_u, double* s_w, do
   __syncthreads();
 }
 
-template <int DIM, int N, int EPW, bool SC, bool NODAL>
+// Minimum waves per SIMD the register allocator must allow (1 = no limit).
+#ifndef DGSEM_WAVES_3D
+#define DGSEM_WAVES_3D 1
+#endif
+#ifndef DGSEM_WAVES_2D
+#define DGSEM_WAVES_2D 1
+#endif
+template <int DIM, int N>
+constexpr int min_waves_per_simd() {
+  return DIM == 3 ? DGSEM_WAVES_3D : DGSEM_WAVES_2D;
+}
+
+#ifndef DGSEM_PAIR_FENCE_MIN_N
+#define DGSEM_PAIR_FENCE_MIN_N 99
+#endif
+template <int DIM, int N>
+constexpr bool pair_fence() {
+  return N >= DGSEM_PAIR_FENCE_MIN_N;
+}
+
+// SF / FF: surface and subcell-FV flux kinds (kFluxLLF, kFluxHLLC) as
+// template parameters, so a kernel only carries the register footprint of
+// the fluxes it uses.
+template <int DIM, int N, int EPW, bool SC, bool NODAL, int SF, int FF>
 __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
+    __attribute__((amdgpu_waves_per_eu(min_waves_per_simd<DIM, N>())))
     rhs_kernel(const KParams p) {
   using S = Shape<DIM, N>;
   constexpr int n = S::n, NV = S::NV, NN = S::NN, FN = S::FN, L = S::LINES, NP = S::NP;
   constexpr int BLOCK = EPW * L;
+  // s_p holds the node primitives during the flux phase; before it, the
+  // indicator's modal sweeps use it as scratch and after it, once every line
+  // thread is done reading primitives, the per-direction sums (s_du).
+  static_assert(NP >= NV && NP >= 2, "s_p must be able to hold s_du and scratch");
   __shared__ double s_u[EPW * NV * NN];
   __shared__ double s_p[EPW * NP * NN];
-  __shared__ double s_du[EPW * NV * NN];
   __shared__ double s_alpha[EPW];
+  double* const s_du = s_p;
 
   const int tid = threadIdx.x;
   const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -178,7 +208,9 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
   if (tid < EPW) s_alpha[tid] = 0.0;
   __syncthreads();
 
-  // 2. per-node primitives
+  // 2. blending factor (indicator sweeps use s_p as scratch), then per-node
+  // primitives
+  if constexpr (SC) block_alpha<DIM, N, EPW, BLOCK>(p, s_u, s_p, s_alpha, nvalid);
   for (int i = tid; i < EPW * NN; i += BLOCK) {
     const int slot = i / NN, node = i % NN;
     if (slot >= nvalid) continue;
@@ -190,12 +222,10 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
     for (int k = 0; k < DIM; ++k) d[(1 + k) * NN] = q.v[k];
     d[(DIM + 1) * NN] = q.beta;
     d[(DIM + 2) * NN] = q.kin;
+    d[(DIM + 3) * NN] = q.p;
+    d[(DIM + 4) * NN] = q.c;
   }
-  if constexpr (SC) {
-    block_alpha<DIM, N, EPW, BLOCK>(p, s_u, s_du, s_alpha, nvalid);
-  } else {
-    __syncthreads();
-  }
+  __syncthreads();
 
   // 3. line threads
   const int slot = tid / L, rem = tid % L, dir = rem / FN, line = rem % FN;
@@ -232,19 +262,26 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
 
     // flux differencing along the line: D_split is zero on the diagonal
     const double* pr = s_p + slot * NP * NN;
-    NodePrims<DIM> q[n];
+    auto load_prims = [&](int dof) {
+      NodePrims<DIM> q;
+      q.rho = pr[dof];
 #pragma unroll
-    for (int k = 0; k < n; ++k) {
-      q[k].rho = pr[dofs[k]];
+      for (int c = 0; c < DIM; ++c) q.v[c] = pr[(1 + c) * NN + dof];
+      q.beta = pr[(DIM + 1) * NN + dof];
+      q.kin = pr[(DIM + 2) * NN + dof];
+      q.p = 0.0;
+      q.c = 0.0;
+      return q;
+    };
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) q[k].v[c] = pr[(1 + c) * NN + dofs[k]];
-      q[k].beta = pr[(DIM + 1) * NN + dofs[k]];
-      q[k].kin = pr[(DIM + 2) * NN + dofs[k]];
-    }
-#pragma unroll
-    for (int a = 0; a < n; ++a) {
+    for (int a = 0; a < n - 1; ++a) {
+      const NodePrims<DIM> qa = load_prims(dofs[a]);
 #pragma unroll
       for (int b = a + 1; b < n; ++b) {
+        // A scheduling barrier per pair keeps the unrolled pair loop from
+        // interleaving pairs, which bounds the live registers.
+        if constexpr (pair_fence<DIM, N>()) __builtin_amdgcn_sched_barrier(0);
+        const NodePrims<DIM> qb = load_prims(dofs[b]);
         double nrm[DIM], f[NV];
         if constexpr (NODAL) {
 #pragma unroll
@@ -253,7 +290,7 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
 #pragma unroll
           for (int c = 0; c < DIM; ++c) nrm[c] = ja[0][c];
         }
-        flux_chandrashekar<DIM>(p.inv_gm1, q[a], q[b], nrm, f);
+        flux_chandrashekar<DIM>(p.inv_gm1, qa, qb, nrm, f);
         const double wa = coef * p.Ds[a][b], wb = coef * p.Ds[b][a];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -280,7 +317,10 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
           } else {
             for (int c = 0; c < DIM; ++c) nrm[c] = ja[0][c];
           }
-          surface_flux<DIM>(p.fv_flux, p.gamma, ul, ur, nrm, f);
+          if constexpr (FF == kFluxHLLC)
+            flux_hllc<DIM>(p.gamma, ul, ur, nrm, f);
+          else
+            flux_llf<DIM>(p.gamma, ul, ur, nrm, f);
           const double wl = alpha * p.inv_w[fi - 1], wr = alpha * p.inv_w[fi];
           for (int v = 0; v < NV; ++v) {
             acc[fi - 1][v] += wl * f[v];
@@ -317,7 +357,26 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
         const double sg = p.invJ[eu] >= 0.0 ? 1.0 : -1.0;
         for (int c = 0; c < DIM; ++c) nrm[c] = sg * p.Ja[(eu * DIM + dir) * DIM + c];
       }
-      surface_flux<DIM>(p.surface_flux, p.gamma, ul, ur, nrm, f);
+      if constexpr (SF == kFluxLLF) {
+        // own side from the hoisted node quantities, the neighbour's trace
+        // with one reciprocal and one square root
+        const int dof_own = side == 0 ? dof_up : dof_lo;
+        NodePrims<DIM> qo;
+        qo.rho = pr[dof_own];
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) qo.v[c] = pr[(1 + c) * NN + dof_own];
+        qo.kin = pr[(DIM + 2) * NN + dof_own];
+        qo.p = pr[(DIM + 3) * NN + dof_own];
+        qo.c = pr[(DIM + 4) * NN + dof_own];
+        qo.beta = 0.0;
+        const NodePrims<DIM> qn = face_prims<DIM>(p.gamma, side == 0 ? ul : ur);
+        if (side == 0)
+          flux_llf_prims<DIM>(ul, qn, ur, qo, nrm, f);
+        else
+          flux_llf_prims<DIM>(ul, qo, ur, qn, nrm, f);
+      } else {
+        flux_hllc<DIM>(p.gamma, ul, ur, nrm, f);
+      }
       if (side == 0) {
         for (int v = 0; v < NV; ++v) acc[0][v] -= f[v] * p.lhat0;
       } else {
@@ -326,8 +385,9 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
     }
   }
 
-  // 4. per-direction contributions into LDS, one direction at a time
-  __syncthreads();  // s_du may still hold indicator scratch
+  // 4. per-direction contributions into LDS, one direction at a time; s_du
+  // aliases the primitives, so every thread must be done reading them first
+  __syncthreads();
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
     if (valid && dir == d) {
@@ -349,8 +409,12 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
     const int64_t ee = e0 + sl;
     const double fac = NODAL ? -p.invJ[ee * NN + node] : -p.invJ[ee];
     double r[NV];
-    for (int v = 0; v < NV; ++v) r[v] = s_du[(sl * NV + v) * NN + node] * fac;
     const int64_t g = ee * NV * NN + node;
+    if (p.visc) {
+      for (int v = 0; v < NV; ++v) r[v] = (s_du[(sl * NV + v) * NN + node] + p.visc[g + v * NN]) * fac;
+    } else {
+      for (int v = 0; v < NV; ++v) r[v] = s_du[(sl * NV + v) * NN + node] * fac;
+    }
     if (p.mode == kModeRhs) {
       for (int v = 0; v < NV; ++v) p.out[g + v * NN] = r[v];
     } else {
@@ -376,77 +440,39 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
   }
 }
 
-template <int DIM, int N, bool SC, bool NODAL>
+template <int DIM, int N, bool SC, bool NODAL, int SF, int FF>
 hipError_t launch_one(const KParams& p, hipStream_t stream) {
   constexpr int EPW = elems_per_block<DIM, N>();
   constexpr int BLOCK = EPW * Shape<DIM, N>::LINES;
   const int64_t nb = (p.E + EPW - 1) / EPW;
   if (nb <= 0) return hipSuccess;
   if (nb > INT32_MAX) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((rhs_kernel<DIM, N, EPW, SC, NODAL>), dim3((unsigned)nb), dim3(BLOCK), 0,
-                     stream, p);
+  hipLaunchKernelGGL((rhs_kernel<DIM, N, EPW, SC, NODAL, SF, FF>), dim3((unsigned)nb), dim3(BLOCK),
+                     0, stream, p);
   return hipGetLastError();
+}
+
+template <int DIM, int N, bool SC, bool NODAL>
+hipError_t launch_flux(const KParams& p, hipStream_t s) {
+  const bool shllc = p.surface_flux == kFluxHLLC;
+  if constexpr (SC) {
+    const bool fhllc = p.fv_flux == kFluxHLLC;
+    if (shllc)
+      return fhllc ? launch_one<DIM, N, SC, NODAL, kFluxHLLC, kFluxHLLC>(p, s)
+                   : launch_one<DIM, N, SC, NODAL, kFluxHLLC, kFluxLLF>(p, s);
+    return fhllc ? launch_one<DIM, N, SC, NODAL, kFluxLLF, kFluxHLLC>(p, s)
+                 : launch_one<DIM, N, SC, NODAL, kFluxLLF, kFluxLLF>(p, s);
+  } else {
+    return shllc ? launch_one<DIM, N, SC, NODAL, kFluxHLLC, kFluxLLF>(p, s)
+                 : launch_one<DIM, N, SC, NODAL, kFluxLLF, kFluxLLF>(p, s);
+  }
 }
 
 template <int DIM, int N>
 hipError_t launch_dn(bool sc, bool nodal, const KParams& p, hipStream_t s) {
-  if (sc) return nodal ? launch_one<DIM, N, true, true>(p, s) : launch_one<DIM, N, true, false>(p, s);
-  return nodal ? launch_one<DIM, N, false, true>(p, s) : launch_one<DIM, N, false, false>(p, s);
+  if (sc) return nodal ? launch_flux<DIM, N, true, true>(p, s) : launch_flux<DIM, N, true, false>(p, s);
+  return nodal ? launch_flux<DIM, N, false, true>(p, s) : launch_flux<DIM, N, false, false>(p, s);
 }
 
-__global__ void pack_kernel(const double* __restrict__ src, double* __restrict__ dst, int64_t E,
-                            int nn, int nv, int to_internal) {
-  const int64_t total = E * nn * nv;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    // i enumerates the internal layout [e][v][dof]
-    const int64_t e = i / ((int64_t)nn * nv);
-    const int r = (int)(i % ((int64_t)nn * nv));
-    const int v = r / nn, dof = r % nn;
-    const int64_t j = (e * nn + dof) * nv + v;  // reference layout [e][dof][v]
-    if (to_internal)
-      dst[i] = src[j];
-    else
-      dst[j] = src[i];
-  }
-}
-
-}  // namespace
-
-hipError_t launch_rhs(int dim, int N, bool sc, bool nodal, const KParams& p, hipStream_t s) {
-  if (dim == 2) {
-    switch (N) {
-      case 1: return launch_dn<2, 1>(sc, nodal, p, s);
-      case 2: return launch_dn<2, 2>(sc, nodal, p, s);
-      case 3: return launch_dn<2, 3>(sc, nodal, p, s);
-      case 4: return launch_dn<2, 4>(sc, nodal, p, s);
-      case 5: return launch_dn<2, 5>(sc, nodal, p, s);
-      case 7: return launch_dn<2, 7>(sc, nodal, p, s);
-      default: return hipErrorInvalidValue;
-    }
-  }
-  if (dim == 3) {
-    switch (N) {
-      case 2: return launch_dn<3, 2>(sc, nodal, p, s);
-      case 3: return launch_dn<3, 3>(sc, nodal, p, s);
-      case 4: return launch_dn<3, 4>(sc, nodal, p, s);
-      case 5: return launch_dn<3, 5>(sc, nodal, p, s);
-      case 7: return launch_dn<3, 7>(sc, nodal, p, s);
-      default: return hipErrorInvalidValue;
-    }
-  }
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_pack(const double* src, double* dst, int64_t E, int nn, int nv,
-                       bool to_internal, hipStream_t stream) {
-  const int64_t total = E * nn * nv;
-  if (total <= 0) return hipSuccess;
-  int64_t blocks = (total + 255) / 256;
-  if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(pack_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, src, dst, E, nn,
-                     nv, to_internal ? 1 : 0);
-  return hipGetLastError();
-}
-
+}  // namespace detail
 }  // namespace dgsem
