@@ -13,12 +13,14 @@ FLUX_CHANDRASHEKAR = 0
 FLUX_LLF, FLUX_HLLC = 0, 1
 GEOMETRY_NODAL, GEOMETRY_AFFINE = 0, 1
 BC_PERIODIC = 0
+LAYOUT_RIGHT, LAYOUT_LEFT = 0, 1
 
 EXPORTED = (
     "dgsem_abi_version", "dgsem_config_default", "dgsem_create", "dgsem_destroy",
     "dgsem_last_error", "dgsem_sizes", "dgsem_set_indicator_parameters",
     "dgsem_node_coordinates", "dgsem_contravariant_vectors", "dgsem_inverse_jacobian",
-    "dgsem_pack_state", "dgsem_unpack_state", "dgsem_rhs", "dgsem_get_alpha",
+    "dgsem_pack_state", "dgsem_unpack_state", "dgsem_pack_state_layout",
+    "dgsem_unpack_state_layout", "dgsem_rhs", "dgsem_get_alpha",
     "dgsem_ssprk3_stage", "dgsem_ssprk3_steps", "dgsem_synchronize",
 )
 
@@ -65,6 +67,8 @@ def load(path: str | None = None):
         getattr(L, name).argtypes = [vp, vp]
     L.dgsem_pack_state.argtypes = [vp, vp, vp]
     L.dgsem_unpack_state.argtypes = [vp, vp, vp]
+    L.dgsem_pack_state_layout.argtypes = [vp, C.c_int, vp, vp]
+    L.dgsem_unpack_state_layout.argtypes = [vp, C.c_int, vp, vp]
     L.dgsem_rhs.argtypes = [vp, vp, vp, C.c_double]
     L.dgsem_get_alpha.argtypes = [vp, vp]
     L.dgsem_ssprk3_stage.argtypes = [vp, C.c_int, C.c_double, vp, vp, vp]

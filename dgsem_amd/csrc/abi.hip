@@ -372,6 +372,24 @@ int dgsem_unpack_state(dgsem_handle* h, const double* internal, double* ref) {
   return e == hipSuccess ? DGSEM_OK : hip_fail(h, e, "unpack");
 }
 
+int dgsem_pack_state_layout(dgsem_handle* h, int layout, const double* ref, double* internal) {
+  if (!h || !h->d_Ja || !ref || !internal) return DGSEM_E_INVALID;
+  if (layout == DGSEM_LAYOUT_RIGHT) return dgsem_pack_state(h, ref, internal);
+  if (layout != DGSEM_LAYOUT_LEFT) return fail(h, DGSEM_E_INVALID, "unknown layout");
+  hipError_t e = dgsem::launch_pack_left(ref, internal, h->cfg.cells, h->nn, h->nvars, true,
+                                         h->stream);
+  return e == hipSuccess ? DGSEM_OK : hip_fail(h, e, "pack");
+}
+
+int dgsem_unpack_state_layout(dgsem_handle* h, int layout, const double* internal, double* ref) {
+  if (!h || !h->d_Ja || !ref || !internal) return DGSEM_E_INVALID;
+  if (layout == DGSEM_LAYOUT_RIGHT) return dgsem_unpack_state(h, internal, ref);
+  if (layout != DGSEM_LAYOUT_LEFT) return fail(h, DGSEM_E_INVALID, "unknown layout");
+  hipError_t e = dgsem::launch_pack_left(internal, ref, h->cfg.cells, h->nn, h->nvars, false,
+                                         h->stream);
+  return e == hipSuccess ? DGSEM_OK : hip_fail(h, e, "unpack");
+}
+
 int dgsem_rhs(dgsem_handle* h, const double* u, double* du, double /*t*/) {
   if (!h || !h->d_Ja) return DGSEM_E_INVALID;
   if (!u || !du || u == du) return fail(h, DGSEM_E_INVALID, "u and du must be distinct device arrays");

@@ -64,4 +64,10 @@ hipError_t launch_rhs(int dim, int N, bool shock_capturing, bool nodal, const KP
 hipError_t launch_pack(const double* src, double* dst, int64_t E, int nn, int nv,
                        bool to_internal, hipStream_t stream);
 
+// Same between a column-major (LayoutLeft) reference view u(ie, je[, ke],
+// dof, var) over cells[0] x cells[1] x cells[2] elements and the internal
+// layout.
+hipError_t launch_pack_left(const double* src, double* dst, const int cells[3], int nn, int nv,
+                            bool to_internal, hipStream_t stream);
+
 }  // namespace dgsem

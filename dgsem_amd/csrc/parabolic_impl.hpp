@@ -169,8 +169,7 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) gradient_kernel(
 // Contravariant viscous flux Ja^d . (-F_v) of one node: out[d][v].
 template <int DIM, int NN, bool NODAL>
 __device__ __forceinline__ void contravariant_viscous(const KParams& p, int64_t e, int node,
-                                                      const double* u, int dir_only,
-                                                      double out[DIM][DIM + 2]) {
+                                                      const double* u, double out[DIM][DIM + 2]) {
   constexpr int NV = DIM + 2;
   double q[NV], g[DIM][NV], fv[DIM][NV];
   grad_vars<DIM>(p.gamma, u, q);
@@ -178,7 +177,6 @@ __device__ __forceinline__ void contravariant_viscous(const KParams& p, int64_t 
     for (int v = 0; v < NV; ++v) g[c][v] = p.grad[((e * DIM + c) * NV + v) * NN + node];
   for (int c = 0; c < DIM; ++c) viscous_flux_dir<DIM>(p.mu, p.kappa, q, g, c, fv[c]);
   for (int d = 0; d < DIM; ++d) {
-    if (dir_only >= 0 && d != dir_only) continue;
     double ja[DIM];
     for (int c = 0; c < DIM; ++c) ja[c] = metric<NODAL, DIM, NN>(p, e, d, c, node);
     for (int v = 0; v < NV; ++v) {
@@ -207,7 +205,7 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) viscous_kernel(c
     const int64_t e = e0 + sl;
     double s[NV], out[DIM][NV];
     for (int v = 0; v < NV; ++v) s[v] = p.u[(e * NV + v) * NN + node];
-    contravariant_viscous<DIM, NN, NODAL>(p, e, node, s, -1, out);
+    contravariant_viscous<DIM, NN, NODAL>(p, e, node, s, out);
     for (int d = 0; d < DIM; ++d)
       for (int v = 0; v < NV; ++v) s_f[((sl * DIM + d) * NV + v) * NN + node] = out[d][v];
   }
@@ -239,11 +237,26 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) viscous_kernel(c
     // upper face: the upper neighbour is primary
     {
       const int64_t nb = p.nbr[e * 2 * DIM + 2 * dir + 1];
-      double s[NV], out[DIM][NV];
+      double s[NV], q[NV], g[DIM][NV], fv[DIM][NV];
       for (int v = 0; v < NV; ++v) s[v] = p.u[(nb * NV + v) * NN + dofs[0]];
-      contravariant_viscous<DIM, NN, NODAL>(p, nb, dofs[0], s, dir, out);
+      grad_vars<DIM>(p.gamma, s, q);
+#pragma unroll
+      for (int c = 0; c < DIM; ++c)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) g[c][v] = p.grad[((nb * DIM + c) * NV + v) * NN + dofs[0]];
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) viscous_flux_dir<DIM>(p.mu, p.kappa, q, g, c, fv[c]);
+      double ja[DIM];
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) ja[c] = metric<NODAL, DIM, NN>(p, nb, dir, c, dofs[0]);
       const double sg = inv_jac<NODAL, NN>(p, nb, dofs[0]) >= 0.0 ? 1.0 : -1.0;
-      for (int v = 0; v < NV; ++v) acc[n - 1][v] += sg * out[dir][v] * p.lhatN;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        double r = 0.0;
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) r += ja[c] * -fv[c][v];
+        acc[n - 1][v] += sg * r * p.lhatN;
+      }
     }
   }
 #pragma unroll

@@ -25,6 +25,33 @@ __global__ void pack_kernel(const double* __restrict__ src, double* __restrict__
   }
 }
 
+// Column-major (LayoutLeft) view u(ie, je[, ke], dof, var): ie fastest.  The
+// thread index enumerates that layout so that its side is contiguous; the
+// element's internal index is (ie * ny + je) * nz + ke.
+__global__ void pack_left_kernel(const double* __restrict__ src, double* __restrict__ dst, int nx,
+                                 int ny, int nz, int nn, int nv, int to_internal) {
+  const int64_t E = (int64_t)nx * ny * nz;
+  const int64_t total = E * nn * nv;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t r = i;
+    const int ie = (int)(r % nx);
+    r /= nx;
+    const int je = (int)(r % ny);
+    r /= ny;
+    const int ke = (int)(r % nz);
+    r /= nz;
+    const int dof = (int)(r % nn);
+    const int v = (int)(r / nn);
+    const int64_t e = ((int64_t)ie * ny + je) * nz + ke;
+    const int64_t j = (e * nv + v) * nn + dof;  // internal layout [e][v][dof]
+    if (to_internal)
+      dst[j] = src[i];
+    else
+      dst[i] = src[j];
+  }
+}
+
 }  // namespace
 
 hipError_t launch_rhs(int dim, int N, bool sc, bool nodal, const KParams& p, hipStream_t s) {
@@ -84,6 +111,17 @@ hipError_t launch_pack(const double* src, double* dst, int64_t E, int nn, int nv
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(pack_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, src, dst, E, nn,
                      nv, to_internal ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_left(const double* src, double* dst, const int cells[3], int nn, int nv,
+                            bool to_internal, hipStream_t stream) {
+  const int64_t total = (int64_t)cells[0] * cells[1] * cells[2] * nn * nv;
+  if (total <= 0) return hipSuccess;
+  int64_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(pack_left_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, src, dst,
+                     cells[0], cells[1], cells[2], nn, nv, to_internal ? 1 : 0);
   return hipGetLastError();
 }
 

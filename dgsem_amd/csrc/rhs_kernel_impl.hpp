@@ -38,7 +38,16 @@ struct Shape {
   static constexpr int FN = DIM == 2 ? n : n * n;  // lines per direction
   static constexpr int LINES = DIM * FN;            // line threads per element
   static constexpr int NP = DIM + 5;                // rho, v[DIM], beta, kin, p, c
+  // LDS planes hold one padding slot per row of n nodes, so that line
+  // threads walking rows (node stride n) hit distinct banks.
+  static constexpr int PN = NN + NN / n;
 };
+
+// Padded LDS position of a node (see Shape::PN).
+template <int N>
+__device__ __forceinline__ int lds_pos(int node) {
+  return node + node / (N + 1);
+}
 
 // Elements per workgroup: keeps EPW * LINES near 192 threads.
 template <int DIM, int N>
@@ -85,7 +94,7 @@ __device__ void block_alpha(const KParams& p, const double* s_u, double* s_w, do
   for (int i = tid; i < EPW * NN; i += BLOCK) {
     const int slot = i / NN, node = i % NN;
     double s[NV];
-    for (int v = 0; v < NV; ++v) s[v] = s_u[(slot * NV + v) * NN + node];
+    for (int v = 0; v < NV; ++v) s[v] = s_u[(slot * NV + v) * S::PN + lds_pos<N>(node)];
     A[i] = slot < nvalid ? indicator_variable<DIM>(s) : 0.0;
   }
   __syncthreads();
@@ -162,8 +171,12 @@ __device__ void block_alpha(const KParams& p, const double* s_u, double* s_w, do
 #ifndef DGSEM_WAVES_2D
 #define DGSEM_WAVES_2D 1
 #endif
-template <int DIM, int N>
+// Nodal-metric 3D kernels sit just above the three-waves register budget
+// (170 VGPRs); asking for three waves costs a few spilled registers and
+// measures faster than running two waves.
+template <int DIM, int N, bool NODAL>
 constexpr int min_waves_per_simd() {
+  if constexpr (DIM == 3 && NODAL && N <= 3 && DGSEM_WAVES_3D == 1) return 3;
   return DIM == 3 ? DGSEM_WAVES_3D : DGSEM_WAVES_2D;
 }
 
@@ -180,7 +193,7 @@ constexpr bool pair_fence() {
 // the fluxes it uses.
 template <int DIM, int N, int EPW, bool SC, bool NODAL, int SF, int FF>
 __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
-    __attribute__((amdgpu_waves_per_eu(min_waves_per_simd<DIM, N>())))
+    __attribute__((amdgpu_waves_per_eu(min_waves_per_simd<DIM, N, NODAL>())))
     rhs_kernel(const KParams p) {
   using S = Shape<DIM, N>;
   constexpr int n = S::n, NV = S::NV, NN = S::NN, FN = S::FN, L = S::LINES, NP = S::NP;
@@ -189,8 +202,9 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
   // indicator's modal sweeps use it as scratch and after it, once every line
   // thread is done reading primitives, the per-direction sums (s_du).
   static_assert(NP >= NV && NP >= 2, "s_p must be able to hold s_du and scratch");
-  __shared__ double s_u[EPW * NV * NN];
-  __shared__ double s_p[EPW * NP * NN];
+  constexpr int PN = S::PN;
+  __shared__ double s_u[EPW * NV * PN];
+  __shared__ double s_p[EPW * NP * PN];
   __shared__ double s_alpha[EPW];
   double* const s_du = s_p;
 
@@ -199,11 +213,57 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
   const int64_t e0 = (int64_t)blk * EPW;
   const int nvalid = (int)((p.E - e0) < EPW ? (p.E - e0) : EPW);
 
+  // Line-thread identity, and the global loads the line phase needs (its
+  // metric row, the neighbours' traces at both line ends and the upper
+  // neighbour's metric), issued first so they overlap the LDS staging.
+  const int slot = tid / L, rem = tid % L, dir = rem / FN, line = rem % FN;
+  const bool valid = slot < nvalid;
+  const int64_t e = e0 + slot;
+  int dofs[n], pdofs[n];  // global-memory and padded LDS positions
+#pragma unroll
+  for (int k = 0; k < n; ++k) {
+    dofs[k] = line_dof<DIM, N>(dir, line, k);
+    pdofs[k] = lds_pos<N>(dofs[k]);
+  }
+  double ja[NODAL ? n : 1][DIM];
+  double nb_lo_u[NV], nb_hi_u[NV], nb_ja[DIM];
+  double own_sg = 1.0, nb_sg = 1.0;
+  if (valid) {
+    if constexpr (NODAL) {
+#pragma unroll
+      for (int k = 0; k < n; ++k)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) ja[k][c] = p.Ja[((e * DIM + dir) * DIM + c) * NN + dofs[k]];
+    } else {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) ja[0][c] = p.Ja[(e * DIM + dir) * DIM + c];
+    }
+    const int64_t nb_lo = p.nbr[e * 2 * DIM + 2 * dir], nb_hi = p.nbr[e * 2 * DIM + 2 * dir + 1];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      nb_lo_u[v] = p.u[(nb_lo * NV + v) * NN + dofs[n - 1]];
+      nb_hi_u[v] = p.u[(nb_hi * NV + v) * NN + dofs[0]];
+    }
+    if constexpr (NODAL) {
+      // the upper neighbour's metric row is loaded where it is used (below),
+      // which keeps the register count at three waves per SIMD
+      own_sg = p.invJ[e * NN + dofs[0]] >= 0.0 ? 1.0 : -1.0;
+    } else {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) nb_ja[c] = p.Ja[(nb_hi * DIM + dir) * DIM + c];
+      own_sg = p.invJ[e] >= 0.0 ? 1.0 : -1.0;
+      nb_sg = p.invJ[nb_hi] >= 0.0 ? 1.0 : -1.0;
+    }
+  }
+
   // 1. stage input of the block's elements: one contiguous chunk
   {
     const double* src = p.u + e0 * NV * NN;
     const int cnt = nvalid * NV * NN;
-    for (int i = tid; i < EPW * NV * NN; i += BLOCK) s_u[i] = i < cnt ? src[i] : 1.0;
+    for (int i = tid; i < EPW * NV * NN; i += BLOCK) {
+      const int row = i / NN, node = i - row * NN;  // row = slot * NV + var
+      s_u[row * PN + lds_pos<N>(node)] = i < cnt ? src[i] : 1.0;
+    }
   }
   if (tid < EPW) s_alpha[tid] = 0.0;
   __syncthreads();
@@ -215,43 +275,27 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
     const int slot = i / NN, node = i % NN;
     if (slot >= nvalid) continue;
     double s[NV];
-    for (int v = 0; v < NV; ++v) s[v] = s_u[(slot * NV + v) * NN + node];
+    const int pn = lds_pos<N>(node);
+    for (int v = 0; v < NV; ++v) s[v] = s_u[(slot * NV + v) * PN + pn];
     const NodePrims<DIM> q = node_prims<DIM>(p.gamma, s);
-    double* d = s_p + slot * NP * NN + node;
+    double* d = s_p + slot * NP * PN + pn;
     d[0] = q.rho;
-    for (int k = 0; k < DIM; ++k) d[(1 + k) * NN] = q.v[k];
-    d[(DIM + 1) * NN] = q.beta;
-    d[(DIM + 2) * NN] = q.kin;
-    d[(DIM + 3) * NN] = q.p;
-    d[(DIM + 4) * NN] = q.c;
+    for (int k = 0; k < DIM; ++k) d[(1 + k) * PN] = q.v[k];
+    d[(DIM + 1) * PN] = q.beta;
+    d[(DIM + 2) * PN] = q.kin;
+    d[(DIM + 3) * PN] = q.p;
+    d[(DIM + 4) * PN] = q.c;
   }
   __syncthreads();
 
   // 3. line threads
-  const int slot = tid / L, rem = tid % L, dir = rem / FN, line = rem % FN;
-  const bool valid = slot < nvalid;
-  const int64_t e = e0 + slot;
   double acc[n][NV];
 #pragma unroll
   for (int k = 0; k < n; ++k)
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[k][v] = 0.0;
-  int dofs[n];
-#pragma unroll
-  for (int k = 0; k < n; ++k) dofs[k] = line_dof<DIM, N>(dir, line, k);
 
   if (valid) {
-    // metric row Ja^dir at the line's nodes
-    double ja[NODAL ? n : 1][DIM];
-    if constexpr (NODAL) {
-#pragma unroll
-      for (int k = 0; k < n; ++k)
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) ja[k][c] = p.Ja[((e * DIM + dir) * DIM + c) * NN + dofs[k]];
-    } else {
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) ja[0][c] = p.Ja[(e * DIM + dir) * DIM + c];
-    }
     double alpha = 0.0, coef = 1.0;
     bool troubled = false;
     if constexpr (SC) {
@@ -261,27 +305,27 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
     }
 
     // flux differencing along the line: D_split is zero on the diagonal
-    const double* pr = s_p + slot * NP * NN;
-    auto load_prims = [&](int dof) {
+    const double* pr = s_p + slot * NP * PN;
+    auto load_prims = [&](int pdof) {
       NodePrims<DIM> q;
-      q.rho = pr[dof];
+      q.rho = pr[pdof];
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) q.v[c] = pr[(1 + c) * NN + dof];
-      q.beta = pr[(DIM + 1) * NN + dof];
-      q.kin = pr[(DIM + 2) * NN + dof];
+      for (int c = 0; c < DIM; ++c) q.v[c] = pr[(1 + c) * PN + pdof];
+      q.beta = pr[(DIM + 1) * PN + pdof];
+      q.kin = pr[(DIM + 2) * PN + pdof];
       q.p = 0.0;
       q.c = 0.0;
       return q;
     };
 #pragma unroll
     for (int a = 0; a < n - 1; ++a) {
-      const NodePrims<DIM> qa = load_prims(dofs[a]);
+      const NodePrims<DIM> qa = load_prims(pdofs[a]);
 #pragma unroll
       for (int b = a + 1; b < n; ++b) {
         // A scheduling barrier per pair keeps the unrolled pair loop from
         // interleaving pairs, which bounds the live registers.
         if constexpr (pair_fence<DIM, N>()) __builtin_amdgcn_sched_barrier(0);
-        const NodePrims<DIM> qb = load_prims(dofs[b]);
+        const NodePrims<DIM> qb = load_prims(pdofs[b]);
         double nrm[DIM], f[NV];
         if constexpr (NODAL) {
 #pragma unroll
@@ -300,7 +344,7 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
       }
     }
 
-    const double* su = s_u + slot * NV * NN;
+    const double* su = s_u + slot * NV * PN;
     if constexpr (SC) {
       if (troubled) {
         // first-order finite volumes on the Gauss-Lobatto subcells
@@ -308,8 +352,8 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
         for (int fi = 1; fi < n; ++fi) {
           double ul[NV], ur[NV], nrm[DIM], f[NV];
           for (int v = 0; v < NV; ++v) {
-            ul[v] = su[v * NN + dofs[fi - 1]];
-            ur[v] = su[v * NN + dofs[fi]];
+            ul[v] = su[v * PN + pdofs[fi - 1]];
+            ur[v] = su[v * PN + pdofs[fi]];
           }
           if constexpr (NODAL) {
             const double* snp = p.sn + ((((e * DIM + dir) * FN) + line) * (n - 1) + (fi - 1)) * DIM;
@@ -333,53 +377,47 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
     // surface fluxes on the two faces the line ends on.  Each interface flux
     // is evaluated with the upper element's metric at its first node, as the
     // reference's per-element interface kernel does, so both neighbours
-    // compute bit-identical values.
+    // compute bit-identical values.  One code path serves both ends.
+    const int dof_up = pdofs[0], dof_lo = pdofs[n - 1];  // padded LDS positions
+    if constexpr (NODAL) {
+      const int64_t nb_hi = p.nbr[e * 2 * DIM + 2 * dir + 1];
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) nb_ja[c] = p.Ja[((nb_hi * DIM + dir) * DIM + c) * NN + dofs[0]];
+      nb_sg = p.invJ[nb_hi * NN + dofs[0]] >= 0.0 ? 1.0 : -1.0;
+    }
 #pragma unroll 1
     for (int side = 0; side < 2; ++side) {
-      const int64_t nb = p.nbr[e * 2 * DIM + 2 * dir + side];
-      const int64_t eu = side == 0 ? e : nb;  // upper element of the interface
-      const int dof_up = dofs[0];             // its first node on this line
-      const int dof_lo = dofs[n - 1];         // last node of the lower element
+      const bool lower = side == 0;
       double ul[NV], ur[NV], nrm[DIM], f[NV];
+#pragma unroll
       for (int v = 0; v < NV; ++v) {
-        if (side == 0) {
-          ul[v] = p.u[(nb * NV + v) * NN + dof_lo];
-          ur[v] = su[v * NN + dof_up];
-        } else {
-          ul[v] = su[v * NN + dof_lo];
-          ur[v] = p.u[(nb * NV + v) * NN + dof_up];
-        }
+        ul[v] = lower ? nb_lo_u[v] : su[v * PN + dof_lo];
+        ur[v] = lower ? su[v * PN + dof_up] : nb_hi_u[v];
       }
-      if constexpr (NODAL) {
-        const double sg = p.invJ[eu * NN + dof_up] >= 0.0 ? 1.0 : -1.0;
-        for (int c = 0; c < DIM; ++c) nrm[c] = sg * p.Ja[((eu * DIM + dir) * DIM + c) * NN + dof_up];
-      } else {
-        const double sg = p.invJ[eu] >= 0.0 ? 1.0 : -1.0;
-        for (int c = 0; c < DIM; ++c) nrm[c] = sg * p.Ja[(eu * DIM + dir) * DIM + c];
-      }
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) nrm[c] = lower ? own_sg * ja[0][c] : nb_sg * nb_ja[c];
       if constexpr (SF == kFluxLLF) {
         // own side from the hoisted node quantities, the neighbour's trace
         // with one reciprocal and one square root
-        const int dof_own = side == 0 ? dof_up : dof_lo;
+        const int dof_own = lower ? dof_up : dof_lo;
         NodePrims<DIM> qo;
         qo.rho = pr[dof_own];
 #pragma unroll
-        for (int c = 0; c < DIM; ++c) qo.v[c] = pr[(1 + c) * NN + dof_own];
-        qo.kin = pr[(DIM + 2) * NN + dof_own];
-        qo.p = pr[(DIM + 3) * NN + dof_own];
-        qo.c = pr[(DIM + 4) * NN + dof_own];
+        for (int c = 0; c < DIM; ++c) qo.v[c] = pr[(1 + c) * PN + dof_own];
+        qo.kin = pr[(DIM + 2) * PN + dof_own];
+        qo.p = pr[(DIM + 3) * PN + dof_own];
+        qo.c = pr[(DIM + 4) * PN + dof_own];
         qo.beta = 0.0;
-        const NodePrims<DIM> qn = face_prims<DIM>(p.gamma, side == 0 ? ul : ur);
-        if (side == 0)
-          flux_llf_prims<DIM>(ul, qn, ur, qo, nrm, f);
-        else
-          flux_llf_prims<DIM>(ul, qo, ur, qn, nrm, f);
+        const NodePrims<DIM> qn = face_prims<DIM>(p.gamma, lower ? ul : ur);
+        flux_llf_prims<DIM>(ul, lower ? qn : qo, ur, lower ? qo : qn, nrm, f);
       } else {
         flux_hllc<DIM>(p.gamma, ul, ur, nrm, f);
       }
-      if (side == 0) {
+      if (lower) {
+#pragma unroll
         for (int v = 0; v < NV; ++v) acc[0][v] -= f[v] * p.lhat0;
       } else {
+#pragma unroll
         for (int v = 0; v < NV; ++v) acc[n - 1][v] += f[v] * p.lhatN;
       }
     }
@@ -395,7 +433,7 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
       for (int k = 0; k < n; ++k)
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-          const int idx = (slot * NV + v) * NN + dofs[k];
+          const int idx = (slot * NV + v) * PN + pdofs[k];
           s_du[idx] = d == 0 ? acc[k][v] : s_du[idx] + acc[k][v];
         }
     }
@@ -406,21 +444,22 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
   for (int i = tid; i < EPW * NN; i += BLOCK) {
     const int sl = i / NN, node = i % NN;
     if (sl >= nvalid) continue;
+    const int pn = lds_pos<N>(node);
     const int64_t ee = e0 + sl;
     const double fac = NODAL ? -p.invJ[ee * NN + node] : -p.invJ[ee];
     double r[NV];
     const int64_t g = ee * NV * NN + node;
     if (p.visc) {
-      for (int v = 0; v < NV; ++v) r[v] = (s_du[(sl * NV + v) * NN + node] + p.visc[g + v * NN]) * fac;
+      for (int v = 0; v < NV; ++v) r[v] = (s_du[(sl * NV + v) * PN + pn] + p.visc[g + v * NN]) * fac;
     } else {
-      for (int v = 0; v < NV; ++v) r[v] = s_du[(sl * NV + v) * NN + node] * fac;
+      for (int v = 0; v < NV; ++v) r[v] = s_du[(sl * NV + v) * PN + pn] * fac;
     }
     if (p.mode == kModeRhs) {
       for (int v = 0; v < NV; ++v) p.out[g + v * NN] = r[v];
     } else {
       double s[NV];
       for (int v = 0; v < NV; ++v) {
-        const double uk = s_u[(sl * NV + v) * NN + node];
+        const double uk = s_u[(sl * NV + v) * PN + pn];
         s[v] = p.mode == kModeStage2 ? p.c1 * uk + p.c2 * r[v]
                                      : p.c1 * p.u_base[g + v * NN] + p.c2 * uk + p.c3 * r[v];
       }

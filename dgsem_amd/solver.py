@@ -150,6 +150,19 @@ class StructuredSolver:
         self.synchronize()
         return dst.cpu().numpy()
 
+    def pack_layout(self, flat, layout: int):
+        """Flat device buffer of a reference view in `layout` (LAYOUT_RIGHT or
+        LAYOUT_LEFT) -> internal [e][var][dof]."""
+        dst = self.empty()
+        self._check(self._L.dgsem_pack_state_layout(self._h, int(layout), _ptr(flat), _ptr(dst)))
+        return dst
+
+    def unpack_layout(self, u, layout: int):
+        """Internal state -> flat device buffer of a reference view in `layout`."""
+        dst = torch.empty(u.numel(), dtype=torch.float64, device="cuda")
+        self._check(self._L.dgsem_unpack_state_layout(self._h, int(layout), _ptr(u), _ptr(dst)))
+        return dst
+
     def alpha(self):
         a = torch.empty(self.n_elements, dtype=torch.float64, device="cuda")
         self._check(self._L.dgsem_get_alpha(self._h, _ptr(a)))

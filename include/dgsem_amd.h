@@ -119,6 +119,19 @@ int dgsem_inverse_jacobian(const dgsem_handle* h, double* host_out);
 int dgsem_pack_state(dgsem_handle* h, const double* ref_layout, double* internal);
 int dgsem_unpack_state(dgsem_handle* h, const double* internal, double* ref_layout);
 
+/* The reference's solution views are Kokkos::View<T****[*], Device>
+ * (src/base/global_def.hpp:9-37), whose layout is the default execution
+ * space's: LayoutRight on the OpenMP / Serial backends, LayoutLeft on the
+ * HIP / CUDA backends.  These two take the layout explicitly; with
+ * DGSEM_LAYOUT_RIGHT they are dgsem_pack_state / dgsem_unpack_state.  In
+ * LayoutLeft, u(ie, je[, ke], dof, var) has ie fastest and var slowest. */
+#define DGSEM_LAYOUT_RIGHT 0
+#define DGSEM_LAYOUT_LEFT 1
+int dgsem_pack_state_layout(dgsem_handle* h, int layout, const double* ref_layout,
+                            double* internal);
+int dgsem_unpack_state_layout(dgsem_handle* h, int layout, const double* internal,
+                              double* ref_layout);
+
 /* du = L(u): StructuredSolver::calc_rhs(sol, t), structured_solver.hpp:463-480
  * (volume integral, interface flux, boundary conditions, parabolic terms,
  * surface integral, Jacobian).  u and du are device arrays of
