@@ -6,6 +6,6 @@ include/dgsem_amd.h); this package only binds it for Python callers.
 from ._abi import (BC_PERIODIC, EULER, FLUX_CHANDRASHEKAR, FLUX_HLLC, FLUX_LLF,  # noqa: F401
                    GEOMETRY_AFFINE, GEOMETRY_NODAL, LAYOUT_LEFT, LAYOUT_RIGHT, NAVIER_STOKES, SHOCK_CAPTURING, SPLIT_FORM,
                    DgsemError, load)
-from .solver import SSPRK3, SolverConfig, StructuredSolver  # noqa: F401
+from .solver import SSPRK3, Batch, SolverConfig, StructuredSolver  # noqa: F401
 
-__all__ = ["StructuredSolver", "SSPRK3", "SolverConfig", "load", "DgsemError"]
+__all__ = ["StructuredSolver", "SSPRK3", "Batch", "SolverConfig", "load", "DgsemError"]

@@ -22,6 +22,9 @@ EXPORTED = (
     "dgsem_pack_state", "dgsem_unpack_state", "dgsem_pack_state_layout",
     "dgsem_unpack_state_layout", "dgsem_rhs", "dgsem_get_alpha",
     "dgsem_ssprk3_stage", "dgsem_ssprk3_steps", "dgsem_synchronize",
+    "dgsem_batch_create", "dgsem_batch_destroy", "dgsem_batch_last_error", "dgsem_batch_sizes",
+    "dgsem_batch_case", "dgsem_batch_set_dt", "dgsem_batch_rhs", "dgsem_batch_ssprk3_steps",
+    "dgsem_batch_get_tuning", "dgsem_batch_set_tuning", "dgsem_batch_autotune",
 )
 
 
@@ -74,8 +77,20 @@ def load(path: str | None = None):
     L.dgsem_ssprk3_stage.argtypes = [vp, C.c_int, C.c_double, vp, vp, vp]
     L.dgsem_ssprk3_steps.argtypes = [vp, vp, C.c_double, C.c_int, vp, vp]
     L.dgsem_synchronize.argtypes = [vp]
+    L.dgsem_batch_create.argtypes = [C.POINTER(Config), C.c_int, vp, C.POINTER(vp)]
+    L.dgsem_batch_destroy.argtypes = [vp]
+    L.dgsem_batch_last_error.argtypes = [vp]
+    L.dgsem_batch_last_error.restype = C.c_char_p
+    L.dgsem_batch_sizes.argtypes = [vp, ip, i64p]
+    L.dgsem_batch_case.argtypes = [vp, C.c_int, i64p, i64p, C.POINTER(vp)]
+    L.dgsem_batch_set_dt.argtypes = [vp, dp]
+    L.dgsem_batch_rhs.argtypes = [vp, vp, vp]
+    L.dgsem_batch_ssprk3_steps.argtypes = [vp, vp, C.c_int, vp, vp]
+    L.dgsem_batch_get_tuning.argtypes = [vp, ip, ip, ip]
+    L.dgsem_batch_set_tuning.argtypes = [vp, C.c_int, ip, C.c_int]
+    L.dgsem_batch_autotune.argtypes = [vp, vp, vp, C.c_int]
     for name in EXPORTED:
-        if name not in ("dgsem_config_default", "dgsem_last_error"):
+        if name not in ("dgsem_config_default", "dgsem_last_error", "dgsem_batch_last_error"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -14,7 +14,11 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdgsem_amd.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC",
+# -ffp-contract=on: multiply-adds are fused only where the source writes them
+# as one expression, so the fusion (and with it every rounding) is fixed by
+# the source text and the same element arithmetic gives the same bits in
+# every kernel that includes element_body.inc.
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-ffp-contract=on",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-command-line-argument"]
 HOST_FLAGS = ["-std=c++20", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-Wall",
@@ -26,7 +30,9 @@ SOURCES = [
     ("host_setup.cpp", "g++", "host_setup", ()),
     ("rhs_dispatch.hip", "hipcc", "rhs_dispatch", ()),
     ("abi.hip", "hipcc", "abi", ()),
-] + [("rhs_inst.hip", "hipcc", f"rhs_inst_{d}d_n{n}", (f"DGSEM_INST_DIM={d}", f"DGSEM_INST_N={n}"))
+    ("batch.hip", "hipcc", "batch", ()),
+] + [("batch_inst.hip", "hipcc", f"batch_inst_b{b}", (f"DGSEM_BATCH_BLOCK={b}",))
+     for b in (128, 256)] + [("rhs_inst.hip", "hipcc", f"rhs_inst_{d}d_n{n}", (f"DGSEM_INST_DIM={d}", f"DGSEM_INST_N={n}"))
      for d, n in INSTANCES]
 
 

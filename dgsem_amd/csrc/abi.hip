@@ -17,29 +17,9 @@
 #include <vector>
 
 #include "../../include/dgsem_amd.h"
+#include "handle.hpp"
 #include "host_setup.hpp"
 #include "kernels.hpp"
-
-struct dgsem_handle {
-  dgsem_config cfg{};
-  dgsem::BasisTables basis;
-  dgsem::Geometry geo;
-  hipStream_t stream = nullptr;
-  int device = 0;
-  int nvars = 0, nn = 0;
-  int64_t E = 0;
-  // device buffers
-  double* d_Ja = nullptr;
-  double* d_invJ = nullptr;
-  double* d_sn = nullptr;
-  int* d_nbr = nullptr;
-  double* d_Vinv = nullptr;
-  double* d_alpha = nullptr;
-  double* d_grad = nullptr;  // Navier-Stokes gradient [E][dim][var][dof]
-  double* d_visc = nullptr;  // Navier-Stokes viscous RHS part [E][var][dof]
-  dgsem::KParams base{};
-  std::string err;
-};
 
 namespace {
 
@@ -232,6 +212,45 @@ int launch(dgsem_handle* h, dgsem::KParams p) {
 
 }  // namespace
 
+namespace dgsem {
+
+int stage_params(dgsem_handle* h, int stage, double dt, const double* u_base, const double* u_k,
+                 double* u_out, KParams* out) {
+  if (!u_k || !u_out || u_out == u_k) return fail(h, DGSEM_E_INVALID, "u_out must not alias u_k");
+  KParams p = h->base;
+  p.u = u_k;
+  p.out = u_out;
+  // coefficients exactly as SSPRK3::step builds them, SSPRK3.hpp:162-180
+  if (stage == 0) {
+    p.mode = kModeRhs;
+  } else if (stage == 1) {
+    p.mode = kModeStage2;
+    p.c1 = 1.0;
+    p.c2 = dt;
+  } else if (stage == 2 || stage == 3) {
+    if (!u_base) return fail(h, DGSEM_E_INVALID, "u_base required for stages 2 and 3");
+    p.mode = kModeStage3;
+    p.u_base = u_base;
+    if (stage == 2) {
+      p.c1 = 3.0 / 4.0;
+      p.c2 = 1.0 / 4.0;
+      p.c3 = 1.0 / 4.0 * dt;
+    } else {
+      p.c1 = 1.0 / 3.0;
+      p.c2 = 2.0 / 3.0;
+      p.c3 = 2.0 / 3.0 * dt;
+    }
+  } else {
+    return fail(h, DGSEM_E_INVALID, "stage must be 0, 1, 2 or 3");
+  }
+  *out = p;
+  return DGSEM_OK;
+}
+
+int launch_params(dgsem_handle* h, const KParams& p) { return launch(h, p); }
+
+}  // namespace dgsem
+
 extern "C" {
 
 int dgsem_abi_version(void) { return DGSEM_ABI_VERSION; }
@@ -410,32 +429,10 @@ int dgsem_get_alpha(const dgsem_handle* h, double* alpha) {
 int dgsem_ssprk3_stage(dgsem_handle* h, int stage, double dt, const double* u_base,
                        const double* u_k, double* u_out) {
   if (!h || !h->d_Ja) return DGSEM_E_INVALID;
-  if (!u_k || !u_out || u_out == u_k) return fail(h, DGSEM_E_INVALID, "u_out must not alias u_k");
-  dgsem::KParams p = h->base;
-  p.u = u_k;
-  p.out = u_out;
-  // coefficients exactly as SSPRK3::step builds them, SSPRK3.hpp:162-180
-  if (stage == 1) {
-    p.mode = dgsem::kModeStage2;
-    p.c1 = 1.0;
-    p.c2 = dt;
-  } else if (stage == 2 || stage == 3) {
-    if (!u_base) return fail(h, DGSEM_E_INVALID, "u_base required for stages 2 and 3");
-    p.mode = dgsem::kModeStage3;
-    p.u_base = u_base;
-    if (stage == 2) {
-      p.c1 = 3.0 / 4.0;
-      p.c2 = 1.0 / 4.0;
-      p.c3 = 1.0 / 4.0 * dt;
-    } else {
-      p.c1 = 1.0 / 3.0;
-      p.c2 = 2.0 / 3.0;
-      p.c3 = 2.0 / 3.0 * dt;
-    }
-  } else {
-    return fail(h, DGSEM_E_INVALID, "stage must be 1, 2 or 3");
-  }
-  return launch(h, p);
+  if (stage < 1 || stage > 3) return fail(h, DGSEM_E_INVALID, "stage must be 1, 2 or 3");
+  dgsem::KParams p;
+  const int rc = dgsem::stage_params(h, stage, dt, u_base, u_k, u_out, &p);
+  return rc != DGSEM_OK ? rc : launch(h, p);
 }
 
 int dgsem_ssprk3_steps(dgsem_handle* h, double* u, double dt, int nsteps, double* tmp1, double* tmp2) {

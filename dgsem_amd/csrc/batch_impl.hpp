@@ -1,0 +1,125 @@
+// Persistent kernel that advances an ensemble of independent 2D cases of
+// different element grids and polynomial degrees in one launch per RK stage.
+//
+// The host cuts every case into tiles of at most EPW(N) consecutive
+// elements (EPW(N) = BLOCK / LINES(N) element slots fit one workgroup) and
+// sorts the tiles by estimated cost.  Each workgroup walks the tile list
+// with a grid-size stride; a tile names its case, whose KParams (state,
+// geometry, basis and stage coefficients) sit in a device array, and its
+// polynomial degree, which selects the instantiation of the element body.
+// The element body is the same source as the single-case kernel
+// (element_body.inc), so every case gets bit-identical results to its own
+// single-case run.  Degrees 1..5 are built: one kernel holds every degree's
+// body, so its register budget is that of the largest, and the N = 7 body
+// alone would cut the whole kernel to one wave per SIMD.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "rhs_kernel_impl.hpp"
+
+namespace dgsem {
+
+namespace detail {
+
+// LDS doubles one tile of degree N needs with BLOCK threads.
+template <int N, int BLOCK>
+constexpr int batch_lds_doubles() {
+  using S = Shape<2, N>;
+  constexpr int EPW = BLOCK / S::LINES;
+  return EPW * (S::NV + S::NP) * S::PN + EPW;
+}
+
+template <int N, int BLOCK, bool SC, bool NODAL, int SF, int FF>
+__device__ __forceinline__ void batch_tile(const KParams& p, const int64_t e0, const int nvalid,
+                                           double* smem) {
+  constexpr int DIM = 2;
+  using S = Shape<DIM, N>;
+  constexpr int n = S::n, NV = S::NV, NN = S::NN, FN = S::FN, L = S::LINES, NP = S::NP;
+  constexpr int PN = S::PN;
+  constexpr int EPW = BLOCK / L;
+  static_assert(EPW >= 1, "workgroup too small for one element");
+  static_assert(NP >= NV && NP >= 2, "s_p must be able to hold s_du and scratch");
+  double* const s_u = smem;
+  double* const s_p = smem + EPW * NV * PN;
+  double* const s_alpha = s_p + EPW * NP * PN;
+  double* const s_du = s_p;
+  // The thread's roles (line, direction, node positions) do not change from
+  // tile to tile; computed from a plain threadIdx.x the compiler would hoist
+  // them out of the tile loop and keep every degree's copy live in registers
+  // across the whole kernel.  The empty asm makes tid opaque per tile.
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+#include "element_body.inc"
+}
+
+template <int BLOCK, bool SC, bool NODAL, int SF, int FF>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
+    batch_kernel(const BatchTile* __restrict__ tiles, int ntiles,
+                 const KParams* __restrict__ params) {
+  extern __shared__ double smem[];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const BatchTile tile = tiles[t];
+    const KParams& p = params[tile.case_id];
+    switch (tile.polydeg) {
+      case 1: batch_tile<1, BLOCK, SC, NODAL, SF, FF>(p, tile.e0, tile.nvalid, smem); break;
+      case 2: batch_tile<2, BLOCK, SC, NODAL, SF, FF>(p, tile.e0, tile.nvalid, smem); break;
+      case 3: batch_tile<3, BLOCK, SC, NODAL, SF, FF>(p, tile.e0, tile.nvalid, smem); break;
+      case 4: batch_tile<4, BLOCK, SC, NODAL, SF, FF>(p, tile.e0, tile.nvalid, smem); break;
+      case 5: batch_tile<5, BLOCK, SC, NODAL, SF, FF>(p, tile.e0, tile.nvalid, smem); break;
+      default: break;
+    }
+    __syncthreads();  // the next tile reuses the LDS
+  }
+}
+
+template <int BLOCK, bool SC, bool NODAL, int SF, int FF>
+hipError_t launch_batch_t(const BatchLaunch& b, hipStream_t stream) {
+  auto kern = batch_kernel<BLOCK, SC, NODAL, SF, FF>;
+  if (b.probe) {
+    // occupancy query for the host's grid sizing; no launch.  It also raises
+    // the kernel's dynamic-LDS limit to b.lds_bytes, which every later launch
+    // (including ones recorded into a graph) relies on.
+    int nb = 0;
+    hipError_t e = hipFuncSetAttribute((const void*)kern,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)b.lds_bytes);
+    if (e != hipSuccess) return e;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, BLOCK, b.lds_bytes);
+    if (e != hipSuccess) return e;
+    *b.probe = nb;
+    return hipSuccess;
+  }
+  if (b.ntiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(kern, dim3((unsigned)b.grid), dim3(BLOCK), b.lds_bytes, stream, b.tiles,
+                     b.ntiles, b.params);
+  return hipGetLastError();
+}
+
+template <int BLOCK, bool SC, bool NODAL>
+hipError_t launch_batch_flux(const BatchLaunch& b, hipStream_t s) {
+  const bool shllc = b.surface_flux == kFluxHLLC;
+  if constexpr (SC) {
+    const bool fhllc = b.fv_flux == kFluxHLLC;
+    if (shllc)
+      return fhllc ? launch_batch_t<BLOCK, SC, NODAL, kFluxHLLC, kFluxHLLC>(b, s)
+                   : launch_batch_t<BLOCK, SC, NODAL, kFluxHLLC, kFluxLLF>(b, s);
+    return fhllc ? launch_batch_t<BLOCK, SC, NODAL, kFluxLLF, kFluxHLLC>(b, s)
+                 : launch_batch_t<BLOCK, SC, NODAL, kFluxLLF, kFluxLLF>(b, s);
+  } else {
+    return shllc ? launch_batch_t<BLOCK, SC, NODAL, kFluxHLLC, kFluxLLF>(b, s)
+                 : launch_batch_t<BLOCK, SC, NODAL, kFluxLLF, kFluxLLF>(b, s);
+  }
+}
+
+template <int BLOCK>
+hipError_t launch_batch_block(const BatchLaunch& b, hipStream_t s) {
+  if (b.shock_capturing)
+    return b.nodal ? launch_batch_flux<BLOCK, true, true>(b, s)
+                   : launch_batch_flux<BLOCK, true, false>(b, s);
+  return b.nodal ? launch_batch_flux<BLOCK, false, true>(b, s)
+                 : launch_batch_flux<BLOCK, false, false>(b, s);
+}
+
+}  // namespace detail
+}  // namespace dgsem

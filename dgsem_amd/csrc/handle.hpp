@@ -1,0 +1,46 @@
+// Internal definition of the library handle, shared by the C ABI
+// (abi.hip) and the batched-ensemble API (batch.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/dgsem_amd.h"
+#include "host_setup.hpp"
+#include "kernels.hpp"
+
+struct dgsem_handle {
+  dgsem_config cfg{};
+  dgsem::BasisTables basis;
+  dgsem::Geometry geo;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int nvars = 0, nn = 0;
+  int64_t E = 0;
+  // device buffers
+  double* d_Ja = nullptr;
+  double* d_invJ = nullptr;
+  double* d_sn = nullptr;
+  int* d_nbr = nullptr;
+  double* d_Vinv = nullptr;
+  double* d_alpha = nullptr;
+  double* d_grad = nullptr;  // Navier-Stokes gradient [E][dim][var][dof]
+  double* d_visc = nullptr;  // Navier-Stokes viscous RHS part [E][var][dof]
+  dgsem::KParams base{};
+  std::string err;
+};
+
+namespace dgsem {
+
+// The kernel parameters of SSPRK3 stage 1, 2 or 3 (coefficients as
+// SSPRK3::step builds them), or of the plain right-hand side for stage 0
+// (u_k -> u_out = L(u_k)).  Returns DGSEM_OK or an error code with the
+// handle's message set.
+int stage_params(dgsem_handle* h, int stage, double dt, const double* u_base, const double* u_k,
+                 double* u_out, KParams* out);
+
+// Enqueues the kernels for one evaluation with parameters p.
+int launch_params(dgsem_handle* h, const KParams& p);
+
+}  // namespace dgsem

@@ -70,4 +70,35 @@ hipError_t launch_pack(const double* src, double* dst, int64_t E, int nn, int nv
 hipError_t launch_pack_left(const double* src, double* dst, const int cells[3], int nn, int nv,
                             bool to_internal, hipStream_t stream);
 
+// Batched ensembles of 2D cases (batch_impl.hpp).  A tile is up to
+// batch_slots(block, N) consecutive elements of one case.
+struct BatchTile {
+  int32_t case_id;
+  int16_t polydeg;
+  int16_t nvalid;  // elements in this tile
+  int64_t e0;      // first element of the tile within its case
+};
+
+struct BatchLaunch {
+  const BatchTile* tiles;
+  int ntiles;
+  const KParams* params;  // one per case
+  int grid;               // persistent workgroups
+  size_t lds_bytes;       // dynamic LDS: the largest tile footprint in the batch
+  bool shock_capturing, nodal;
+  int surface_flux, fv_flux;
+  int* probe;  // non-null: store the resident workgroups per CU instead of launching
+};
+
+// Workgroup sizes the batch kernel is built for.
+constexpr int kBatchBlocks[] = {128, 256};
+
+template <int BLOCK>
+hipError_t launch_batch_bs(const BatchLaunch& b, hipStream_t stream);
+hipError_t launch_batch(int block, const BatchLaunch& b, hipStream_t stream);
+// Element slots of one tile and its LDS bytes, for a 2D tile of degree N;
+// 0 when (block, N) is not built.
+int batch_slots(int block, int polydeg);
+size_t batch_lds_bytes(int block, int polydeg);
+
 }  // namespace dgsem

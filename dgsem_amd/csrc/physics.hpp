@@ -176,32 +176,28 @@ struct NodePrims {
 template <int DIM>
 DG_HD NodePrims<DIM> node_prims(double gamma, const double* u) {
   NodePrims<DIM> q;
-  const double ir = 1.0 / u[0];
   q.rho = u[0];
   double v2 = 0.0;
-  for (int k = 0; k < DIM; ++k) q.v[k] = u[k + 1] * ir;
+  for (int k = 0; k < DIM; ++k) q.v[k] = u[k + 1] / u[0];
   for (int k = 0; k < DIM; ++k) v2 += q.v[k] * q.v[k];
   q.p = (gamma - 1.0) * (u[DIM + 1] - 0.5 * u[0] * v2);
-  const double ip = 1.0 / q.p;
-  q.beta = 0.5 * u[0] * ip;
+  q.beta = 0.5 * u[0] / q.p;
   q.kin = 0.5 * v2;
-  q.c = sqrt(gamma * q.p * ir);
+  q.c = sqrt(gamma * q.p / u[0]);
   return q;
 }
 
 // Chandrashekar two-point flux from hoisted node quantities,
 // numerical_flux.hpp:436-500 (2D) and :520-594 (3D).  inv_gm1 = 1/(gamma-1).
 //
-// The logarithmic means of rho and beta (math_utils.hpp:8-24) use the series
-// (x + y) / P(f2) with f2 = a/b, a = x(x-2y)+y^2, b = x(x+2y)+y^2, whenever
-// f2 < 1e-4.  Multiplying P(f2) by b^3 gives the homogeneous cubic
-//   Q = ((2/7 a + 2/5 b) a + 2/3 b^2) a + 2 b^3,  mean = (x + y) b^3 / Q,
-// and the branch test becomes a < 1e-4 b.  Then rho_mean, 1/beta_mean and
-// p_mean = (rho_a + rho_b) / (2 (beta_a + beta_b)) all follow from the
-// single reciprocal 1 / (Q_rho (beta_a + beta_b) b_beta^3).  The operand
-// ranges stay inside FP64 for densities and pressures down to 1e-13; results
-// agree with the reference's quotients to a few ulp.  If either mean needs
-// the log branch, both are evaluated exactly as the reference does.
+// The two logarithmic means (math_utils.hpp:8-24), the mean pressure
+// 0.5 rho_avg / beta_avg and 1 / beta_mean need five quotients; in the usual
+// case where both means take the series branch (f2 < 1e-4) they come from
+// two reciprocals:  1/(b_rho b_beta) gives f2 of both means, and
+// 1/(poly_rho s_beta) gives rho_mean = s_rho/poly_rho, 1/beta_mean =
+// poly_beta/s_beta and p_mean = s_rho/(2 s_beta), where s = x + y and
+// b = x(x+2y) + y^2.  Results agree with the quotient form to a few ulp.
+// Otherwise the means are evaluated exactly as the reference does.
 template <int DIM>
 DG_HD void flux_chandrashekar(double inv_gm1, const NodePrims<DIM>& a, const NodePrims<DIM>& b,
                               const double* n, double* f) {
@@ -211,24 +207,23 @@ DG_HD void flux_chandrashekar(double inv_gm1, const NodePrims<DIM>& a, const Nod
     vnb += b.v[k] * n[k];
   }
   const double sr = a.rho + b.rho, sb = a.beta + b.beta;
-  const double ar = a.rho * (a.rho - 2.0 * b.rho) + b.rho * b.rho;
-  const double br = a.rho * (a.rho + 2.0 * b.rho) + b.rho * b.rho;
-  const double ab = a.beta * (a.beta - 2.0 * b.beta) + b.beta * b.beta;
-  const double bb = a.beta * (a.beta + 2.0 * b.beta) + b.beta * b.beta;
+  const double nr = a.rho * (a.rho - 2.0 * b.rho) + b.rho * b.rho;
+  const double dr = a.rho * (a.rho + 2.0 * b.rho) + b.rho * b.rho;
+  const double nb = a.beta * (a.beta - 2.0 * b.beta) + b.beta * b.beta;
+  const double db = a.beta * (a.beta + 2.0 * b.beta) + b.beta * b.beta;
+  const double r1 = 1.0 / (dr * db);
+  const double f2r = nr * (db * r1), f2b = nb * (dr * r1);
+  const double polyr = 2.0 + f2r * (2.0 / 3.0 + f2r * (2.0 / 5.0 + f2r * (2.0 / 7.0)));
+  const double polyb = 2.0 + f2b * (2.0 / 3.0 + f2b * (2.0 / 5.0 + f2b * (2.0 / 7.0)));
   double rho_mean, inv_beta_mean, p_mean;
-  if (ar < 1.0e-4 * br && ab < 1.0e-4 * bb) {
-    const double br2 = br * br, bb2 = bb * bb;
-    const double br3 = br2 * br, bb3 = bb2 * bb;
-    const double qr = ((2.0 / 7.0 * ar + 2.0 / 5.0 * br) * ar + 2.0 / 3.0 * br2) * ar + 2.0 * br3;
-    const double qb = ((2.0 / 7.0 * ab + 2.0 / 5.0 * bb) * ab + 2.0 / 3.0 * bb2) * ab + 2.0 * bb3;
-    const double sb3 = sb * bb3;
-    const double r = 1.0 / (qr * sb3);
-    rho_mean = sr * br3 * (sb3 * r);
-    inv_beta_mean = qb * (qr * r);
-    p_mean = 0.5 * sr * (qr * bb3 * r);
+  if (f2r < 1.0e-4 && f2b < 1.0e-4) {
+    const double r2 = 1.0 / (polyr * sb);
+    rho_mean = sr * (sb * r2);
+    inv_beta_mean = polyb * (polyr * r2);
+    p_mean = 0.5 * sr * (polyr * r2);
   } else {
-    rho_mean = ln_mean(a.rho, b.rho);
-    inv_beta_mean = 1.0 / ln_mean(a.beta, b.beta);
+    rho_mean = f2r < 1.0e-4 ? sr / polyr : (b.rho - a.rho) / log(b.rho / a.rho);
+    inv_beta_mean = f2b < 1.0e-4 ? polyb / sb : log(b.beta / a.beta) / (b.beta - a.beta);
     p_mean = 0.5 * sr / sb;
   }
   const double vsq_avg = a.kin + b.kin;

@@ -3,6 +3,7 @@
 // compiled per (dim, N) from rhs_inst.hip.
 #include <hip/hip_runtime.h>
 
+#include "batch_impl.hpp"
 #include "kernels.hpp"
 
 namespace dgsem {
@@ -123,6 +124,54 @@ hipError_t launch_pack_left(const double* src, double* dst, const int cells[3], 
   hipLaunchKernelGGL(pack_left_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, src, dst,
                      cells[0], cells[1], cells[2], nn, nv, to_internal ? 1 : 0);
   return hipGetLastError();
+}
+
+hipError_t launch_batch(int block, const BatchLaunch& b, hipStream_t stream) {
+  switch (block) {
+    case 128: return launch_batch_bs<128>(b, stream);
+    case 256: return launch_batch_bs<256>(b, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+namespace {
+
+template <int BLOCK>
+int slots_for(int N) {
+  switch (N) {
+    case 1: return BLOCK / detail::Shape<2, 1>::LINES;
+    case 2: return BLOCK / detail::Shape<2, 2>::LINES;
+    case 3: return BLOCK / detail::Shape<2, 3>::LINES;
+    case 4: return BLOCK / detail::Shape<2, 4>::LINES;
+    case 5: return BLOCK / detail::Shape<2, 5>::LINES;
+    default: return 0;
+  }
+}
+
+template <int BLOCK>
+size_t lds_for(int N) {
+  switch (N) {
+    case 1: return sizeof(double) * detail::batch_lds_doubles<1, BLOCK>();
+    case 2: return sizeof(double) * detail::batch_lds_doubles<2, BLOCK>();
+    case 3: return sizeof(double) * detail::batch_lds_doubles<3, BLOCK>();
+    case 4: return sizeof(double) * detail::batch_lds_doubles<4, BLOCK>();
+    case 5: return sizeof(double) * detail::batch_lds_doubles<5, BLOCK>();
+    default: return 0;
+  }
+}
+
+}  // namespace
+
+int batch_slots(int block, int N) {
+  if (block == 128) return slots_for<128>(N);
+  if (block == 256) return slots_for<256>(N);
+  return 0;
+}
+
+size_t batch_lds_bytes(int block, int N) {
+  if (block == 128) return lds_for<128>(N);
+  if (block == 256) return lds_for<256>(N);
+  return 0;
 }
 
 }  // namespace dgsem

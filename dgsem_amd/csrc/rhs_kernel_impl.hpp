@@ -213,270 +213,7 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
   const int64_t e0 = (int64_t)blk * EPW;
   const int nvalid = (int)((p.E - e0) < EPW ? (p.E - e0) : EPW);
 
-  // Line-thread identity, and the global loads the line phase needs (its
-  // metric row, the neighbours' traces at both line ends and the upper
-  // neighbour's metric), issued first so they overlap the LDS staging.
-  const int slot = tid / L, rem = tid % L, dir = rem / FN, line = rem % FN;
-  const bool valid = slot < nvalid;
-  const int64_t e = e0 + slot;
-  int dofs[n], pdofs[n];  // global-memory and padded LDS positions
-#pragma unroll
-  for (int k = 0; k < n; ++k) {
-    dofs[k] = line_dof<DIM, N>(dir, line, k);
-    pdofs[k] = lds_pos<N>(dofs[k]);
-  }
-  double ja[NODAL ? n : 1][DIM];
-  double nb_lo_u[NV], nb_hi_u[NV], nb_ja[DIM];
-  double own_sg = 1.0, nb_sg = 1.0;
-  if (valid) {
-    if constexpr (NODAL) {
-#pragma unroll
-      for (int k = 0; k < n; ++k)
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) ja[k][c] = p.Ja[((e * DIM + dir) * DIM + c) * NN + dofs[k]];
-    } else {
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) ja[0][c] = p.Ja[(e * DIM + dir) * DIM + c];
-    }
-    const int64_t nb_lo = p.nbr[e * 2 * DIM + 2 * dir], nb_hi = p.nbr[e * 2 * DIM + 2 * dir + 1];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      nb_lo_u[v] = p.u[(nb_lo * NV + v) * NN + dofs[n - 1]];
-      nb_hi_u[v] = p.u[(nb_hi * NV + v) * NN + dofs[0]];
-    }
-    if constexpr (NODAL) {
-      // the upper neighbour's metric row is loaded where it is used (below),
-      // which keeps the register count at three waves per SIMD
-      own_sg = p.invJ[e * NN + dofs[0]] >= 0.0 ? 1.0 : -1.0;
-    } else {
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) nb_ja[c] = p.Ja[(nb_hi * DIM + dir) * DIM + c];
-      own_sg = p.invJ[e] >= 0.0 ? 1.0 : -1.0;
-      nb_sg = p.invJ[nb_hi] >= 0.0 ? 1.0 : -1.0;
-    }
-  }
-
-  // 1. stage input of the block's elements: one contiguous chunk
-  {
-    const double* src = p.u + e0 * NV * NN;
-    const int cnt = nvalid * NV * NN;
-    for (int i = tid; i < EPW * NV * NN; i += BLOCK) {
-      const int row = i / NN, node = i - row * NN;  // row = slot * NV + var
-      s_u[row * PN + lds_pos<N>(node)] = i < cnt ? src[i] : 1.0;
-    }
-  }
-  if (tid < EPW) s_alpha[tid] = 0.0;
-  __syncthreads();
-
-  // 2. blending factor (indicator sweeps use s_p as scratch), then per-node
-  // primitives
-  if constexpr (SC) block_alpha<DIM, N, EPW, BLOCK>(p, s_u, s_p, s_alpha, nvalid);
-  for (int i = tid; i < EPW * NN; i += BLOCK) {
-    const int slot = i / NN, node = i % NN;
-    if (slot >= nvalid) continue;
-    double s[NV];
-    const int pn = lds_pos<N>(node);
-    for (int v = 0; v < NV; ++v) s[v] = s_u[(slot * NV + v) * PN + pn];
-    const NodePrims<DIM> q = node_prims<DIM>(p.gamma, s);
-    double* d = s_p + slot * NP * PN + pn;
-    d[0] = q.rho;
-    for (int k = 0; k < DIM; ++k) d[(1 + k) * PN] = q.v[k];
-    d[(DIM + 1) * PN] = q.beta;
-    d[(DIM + 2) * PN] = q.kin;
-    d[(DIM + 3) * PN] = q.p;
-    d[(DIM + 4) * PN] = q.c;
-  }
-  __syncthreads();
-
-  // 3. line threads
-  double acc[n][NV];
-#pragma unroll
-  for (int k = 0; k < n; ++k)
-#pragma unroll
-    for (int v = 0; v < NV; ++v) acc[k][v] = 0.0;
-
-  if (valid) {
-    double alpha = 0.0, coef = 1.0;
-    bool troubled = false;
-    if constexpr (SC) {
-      alpha = s_alpha[slot];
-      troubled = fabs(alpha - 0.0) > p.atol;
-      if (troubled) coef = 1.0 - alpha;
-    }
-
-    // flux differencing along the line: D_split is zero on the diagonal
-    const double* pr = s_p + slot * NP * PN;
-    auto load_prims = [&](int pdof) {
-      NodePrims<DIM> q;
-      q.rho = pr[pdof];
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) q.v[c] = pr[(1 + c) * PN + pdof];
-      q.beta = pr[(DIM + 1) * PN + pdof];
-      q.kin = pr[(DIM + 2) * PN + pdof];
-      q.p = 0.0;
-      q.c = 0.0;
-      return q;
-    };
-#pragma unroll
-    for (int a = 0; a < n - 1; ++a) {
-      const NodePrims<DIM> qa = load_prims(pdofs[a]);
-#pragma unroll
-      for (int b = a + 1; b < n; ++b) {
-        // A scheduling barrier per pair keeps the unrolled pair loop from
-        // interleaving pairs, which bounds the live registers.
-        if constexpr (pair_fence<DIM, N>()) __builtin_amdgcn_sched_barrier(0);
-        const NodePrims<DIM> qb = load_prims(pdofs[b]);
-        double nrm[DIM], f[NV];
-        if constexpr (NODAL) {
-#pragma unroll
-          for (int c = 0; c < DIM; ++c) nrm[c] = 0.5 * (ja[a][c] + ja[b][c]);
-        } else {
-#pragma unroll
-          for (int c = 0; c < DIM; ++c) nrm[c] = ja[0][c];
-        }
-        flux_chandrashekar<DIM>(p.inv_gm1, qa, qb, nrm, f);
-        const double wa = coef * p.Ds[a][b], wb = coef * p.Ds[b][a];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          acc[a][v] += wa * f[v];
-          acc[b][v] += wb * f[v];
-        }
-      }
-    }
-
-    const double* su = s_u + slot * NV * PN;
-    if constexpr (SC) {
-      if (troubled) {
-        // first-order finite volumes on the Gauss-Lobatto subcells
-#pragma unroll
-        for (int fi = 1; fi < n; ++fi) {
-          double ul[NV], ur[NV], nrm[DIM], f[NV];
-          for (int v = 0; v < NV; ++v) {
-            ul[v] = su[v * PN + pdofs[fi - 1]];
-            ur[v] = su[v * PN + pdofs[fi]];
-          }
-          if constexpr (NODAL) {
-            const double* snp = p.sn + ((((e * DIM + dir) * FN) + line) * (n - 1) + (fi - 1)) * DIM;
-            for (int c = 0; c < DIM; ++c) nrm[c] = snp[c];
-          } else {
-            for (int c = 0; c < DIM; ++c) nrm[c] = ja[0][c];
-          }
-          if constexpr (FF == kFluxHLLC)
-            flux_hllc<DIM>(p.gamma, ul, ur, nrm, f);
-          else
-            flux_llf<DIM>(p.gamma, ul, ur, nrm, f);
-          const double wl = alpha * p.inv_w[fi - 1], wr = alpha * p.inv_w[fi];
-          for (int v = 0; v < NV; ++v) {
-            acc[fi - 1][v] += wl * f[v];
-            acc[fi][v] -= wr * f[v];
-          }
-        }
-      }
-    }
-
-    // surface fluxes on the two faces the line ends on.  Each interface flux
-    // is evaluated with the upper element's metric at its first node, as the
-    // reference's per-element interface kernel does, so both neighbours
-    // compute bit-identical values.  One code path serves both ends.
-    const int dof_up = pdofs[0], dof_lo = pdofs[n - 1];  // padded LDS positions
-    if constexpr (NODAL) {
-      const int64_t nb_hi = p.nbr[e * 2 * DIM + 2 * dir + 1];
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) nb_ja[c] = p.Ja[((nb_hi * DIM + dir) * DIM + c) * NN + dofs[0]];
-      nb_sg = p.invJ[nb_hi * NN + dofs[0]] >= 0.0 ? 1.0 : -1.0;
-    }
-#pragma unroll 1
-    for (int side = 0; side < 2; ++side) {
-      const bool lower = side == 0;
-      double ul[NV], ur[NV], nrm[DIM], f[NV];
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        ul[v] = lower ? nb_lo_u[v] : su[v * PN + dof_lo];
-        ur[v] = lower ? su[v * PN + dof_up] : nb_hi_u[v];
-      }
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) nrm[c] = lower ? own_sg * ja[0][c] : nb_sg * nb_ja[c];
-      if constexpr (SF == kFluxLLF) {
-        // own side from the hoisted node quantities, the neighbour's trace
-        // with one reciprocal and one square root
-        const int dof_own = lower ? dof_up : dof_lo;
-        NodePrims<DIM> qo;
-        qo.rho = pr[dof_own];
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) qo.v[c] = pr[(1 + c) * PN + dof_own];
-        qo.kin = pr[(DIM + 2) * PN + dof_own];
-        qo.p = pr[(DIM + 3) * PN + dof_own];
-        qo.c = pr[(DIM + 4) * PN + dof_own];
-        qo.beta = 0.0;
-        const NodePrims<DIM> qn = face_prims<DIM>(p.gamma, lower ? ul : ur);
-        flux_llf_prims<DIM>(ul, lower ? qn : qo, ur, lower ? qo : qn, nrm, f);
-      } else {
-        flux_hllc<DIM>(p.gamma, ul, ur, nrm, f);
-      }
-      if (lower) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) acc[0][v] -= f[v] * p.lhat0;
-      } else {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) acc[n - 1][v] += f[v] * p.lhatN;
-      }
-    }
-  }
-
-  // 4. per-direction contributions into LDS, one direction at a time; s_du
-  // aliases the primitives, so every thread must be done reading them first
-  __syncthreads();
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) {
-    if (valid && dir == d) {
-#pragma unroll
-      for (int k = 0; k < n; ++k)
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          const int idx = (slot * NV + v) * PN + pdofs[k];
-          s_du[idx] = d == 0 ? acc[k][v] : s_du[idx] + acc[k][v];
-        }
-    }
-    __syncthreads();
-  }
-
-  // 5. -1/J and the stage combination
-  for (int i = tid; i < EPW * NN; i += BLOCK) {
-    const int sl = i / NN, node = i % NN;
-    if (sl >= nvalid) continue;
-    const int pn = lds_pos<N>(node);
-    const int64_t ee = e0 + sl;
-    const double fac = NODAL ? -p.invJ[ee * NN + node] : -p.invJ[ee];
-    double r[NV];
-    const int64_t g = ee * NV * NN + node;
-    if (p.visc) {
-      for (int v = 0; v < NV; ++v) r[v] = (s_du[(sl * NV + v) * PN + pn] + p.visc[g + v * NN]) * fac;
-    } else {
-      for (int v = 0; v < NV; ++v) r[v] = s_du[(sl * NV + v) * PN + pn] * fac;
-    }
-    if (p.mode == kModeRhs) {
-      for (int v = 0; v < NV; ++v) p.out[g + v * NN] = r[v];
-    } else {
-      double s[NV];
-      for (int v = 0; v < NV; ++v) {
-        const double uk = s_u[(sl * NV + v) * PN + pn];
-        s[v] = p.mode == kModeStage2 ? p.c1 * uk + p.c2 * r[v]
-                                     : p.c1 * p.u_base[g + v * NN] + p.c2 * uk + p.c3 * r[v];
-      }
-      if (p.floor_value > 0.0) {
-        if (s[0] < p.floor_value) s[0] = p.floor_value;
-        double m2 = 0.0;
-        for (int k = 0; k < DIM; ++k) m2 += s[k + 1] * s[k + 1];
-        const double kin = 0.5 * m2 / s[0];
-        if ((p.gamma - 1.0) * (s[NV - 1] - kin) < p.floor_value)
-          s[NV - 1] = p.floor_value / (p.gamma - 1.0) + kin;
-      }
-      for (int v = 0; v < NV; ++v) p.out[g + v * NN] = s[v];
-    }
-  }
-  if constexpr (SC) {
-    if (tid < nvalid) p.alpha[e0 + tid] = s_alpha[tid];
-  }
+#include "element_body.inc"
 }
 
 template <int DIM, int N, bool SC, bool NODAL, int SF, int FF>

@@ -77,21 +77,27 @@ def _ptr(t) -> int:
 class StructuredSolver:
     """DGSEM right-hand side on one GPU (internal state layout [E][var][dof])."""
 
-    def __init__(self, cfg: SolverConfig, stream=None):
+    def __init__(self, cfg: SolverConfig, stream=None, _handle=None, _owner=None):
         if torch is None or not torch.cuda.is_available():
             raise _abi.DgsemError("a ROCm GPU is required")
         self.cfg = cfg
         self._L = _abi.load()
         self.stream = stream if stream is not None else torch.cuda.current_stream()
-        h = C.c_void_p()
-        rc = self._L.dgsem_create(C.byref(cfg.to_c()), C.c_void_p(self.stream.cuda_stream), C.byref(h))
+        self._owner = _owner  # a Batch that owns the handle, if any
+        if _handle is not None:
+            h = _handle
+        else:
+            h = C.c_void_p()
+            rc = self._L.dgsem_create(C.byref(cfg.to_c()), C.c_void_p(self.stream.cuda_stream),
+                                      C.byref(h))
+            self._h = h
+            if rc != 0:
+                msg = self._L.dgsem_last_error(h).decode() if h.value else "create failed"
+                if h.value:
+                    self._L.dgsem_destroy(h)
+                    self._h = C.c_void_p()
+                raise _abi.DgsemError(f"dgsem_create: {msg} ({rc})")
         self._h = h
-        if rc != 0:
-            msg = self._L.dgsem_last_error(h).decode() if h.value else "create failed"
-            if h.value:
-                self._L.dgsem_destroy(h)
-                self._h = C.c_void_p()
-            raise _abi.DgsemError(f"dgsem_create: {msg} ({rc})")
         E, nn, nv = C.c_int64(), C.c_int(), C.c_int()
         self._check(self._L.dgsem_sizes(h, C.byref(E), C.byref(nn), C.byref(nv)))
         self.n_elements, self.ndofs, self.nvars = E.value, nn.value, nv.value
@@ -99,6 +105,8 @@ class StructuredSolver:
         self.ref_shape = (self.n_elements, self.ndofs, self.nvars)
 
     def __del__(self):
+        if getattr(self, "_owner", None) is not None:
+            return  # the batch destroys its case handles
         if getattr(self, "_h", None) is not None and self._h.value:
             self._L.dgsem_destroy(self._h)
             self._h = C.c_void_p()
@@ -203,3 +211,87 @@ class SSPRK3:
             self.step(u, dt)
             self.time += dt
             self.iter += 1
+
+
+class Batch:
+    """Ensemble of independent 2D Euler cases advanced together
+    (dgsem_batch_* in include/dgsem_amd.h).  States live in arenas of
+    `total` doubles; case i is the slice `view(arena, i)` in the internal
+    layout of `case(i)`, a StructuredSolver over the case's own handle."""
+
+    def __init__(self, cfgs: Sequence[SolverConfig], stream=None):
+        if torch is None or not torch.cuda.is_available():
+            raise _abi.DgsemError("a ROCm GPU is required")
+        self._L = _abi.load()
+        self.cfgs = list(cfgs)
+        self.stream = stream if stream is not None else torch.cuda.current_stream()
+        arr = (_abi.Config * len(self.cfgs))(*[c.to_c() for c in self.cfgs])
+        b = C.c_void_p()
+        rc = self._L.dgsem_batch_create(arr, len(self.cfgs), C.c_void_p(self.stream.cuda_stream),
+                                        C.byref(b))
+        self._b = b
+        if rc != 0:
+            msg = self._L.dgsem_batch_last_error(b).decode() if b.value else "create failed"
+            if b.value:
+                self._L.dgsem_batch_destroy(b)
+                self._b = C.c_void_p()
+            raise _abi.DgsemError(f"dgsem_batch_create: {msg} ({rc})")
+        n, total = C.c_int(), C.c_int64()
+        self._check(self._L.dgsem_batch_sizes(b, C.byref(n), C.byref(total)))
+        self.ncases, self.total = n.value, total.value
+        self.offsets, self.counts, self._cases = [], [], []
+        for i in range(self.ncases):
+            off, cnt, h = C.c_int64(), C.c_int64(), C.c_void_p()
+            self._check(self._L.dgsem_batch_case(b, i, C.byref(off), C.byref(cnt), C.byref(h)))
+            self.offsets.append(off.value)
+            self.counts.append(cnt.value)
+            self._cases.append(StructuredSolver(self.cfgs[i], self.stream, _handle=h, _owner=self))
+
+    def __del__(self):
+        if getattr(self, "_b", None) is not None and self._b.value:
+            self._cases = []
+            self._L.dgsem_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise _abi.DgsemError(f"{self._L.dgsem_batch_last_error(self._b).decode()} ({rc})")
+
+    def case(self, i: int) -> StructuredSolver:
+        return self._cases[i]
+
+    def empty(self):
+        return torch.empty(self.total, dtype=torch.float64, device="cuda")
+
+    def view(self, arena, i: int):
+        """Case i's slice of an arena, shaped like the case's internal state."""
+        o, c = self.offsets[i], self.counts[i]
+        return arena[o:o + c].view(self._cases[i].shape)
+
+    def set_dt(self, dt: Sequence[float]) -> None:
+        a = np.ascontiguousarray(dt, dtype=np.float64)
+        if a.shape != (self.ncases,):
+            raise ValueError("one dt per case")
+        self._check(self._L.dgsem_batch_set_dt(self._b, a.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def rhs(self, u, du) -> None:
+        self._check(self._L.dgsem_batch_rhs(self._b, _ptr(u), _ptr(du)))
+
+    def ssprk3_steps(self, u, nsteps: int, tmp1, tmp2) -> None:
+        self._check(self._L.dgsem_batch_ssprk3_steps(self._b, _ptr(u), int(nsteps), _ptr(tmp1),
+                                                     _ptr(tmp2)))
+
+    def tuning(self) -> dict:
+        blk, grid = C.c_int(), C.c_int()
+        tiles = (C.c_int * 8)()
+        self._check(self._L.dgsem_batch_get_tuning(self._b, C.byref(blk), tiles, C.byref(grid)))
+        return {"block": blk.value, "tile_elems": list(tiles), "grid": grid.value}
+
+    def set_tuning(self, block: int, tile_elems: Sequence[int] | None = None, grid: int = 0) -> None:
+        tiles = (C.c_int * 8)(*(list(tile_elems) if tile_elems is not None else [0] * 8))
+        self._check(self._L.dgsem_batch_set_tuning(self._b, int(block), tiles, int(grid)))
+
+    def autotune(self, u, scratch, reps: int = 5) -> dict:
+        self._check(self._L.dgsem_batch_autotune(self._b, _ptr(u), _ptr(scratch), int(reps)))
+        return self.tuning()
+

@@ -163,6 +163,48 @@ int dgsem_ssprk3_steps(dgsem_handle* h, double* u, double dt, int nsteps, double
 /* Waits for all work on the handle's stream. */
 int dgsem_synchronize(dgsem_handle* h);
 
+/* ---------------------------------------------------------------------
+ * Batched ensembles.  The reference has no ensemble API: its drivers run
+ * one StructuredSolver per process, one case after another.  A batch
+ * advances many independent 2D Euler cases (different element grids and
+ * polynomial degrees; one volume integral, flux pair and geometry mode for
+ * all) with one persistent kernel launch per SSPRK3 stage, the three stages
+ * of a step recorded once into a HIP graph.  Each case gives bit-identical
+ * results to its own single-case handle (dgsem_ssprk3_steps).
+ *
+ * A batch's state lives in caller-provided device "arenas" of `total`
+ * doubles: case i occupies [offset_i, offset_i + count_i) in the internal
+ * layout of its own handle.
+ * --------------------------------------------------------------------- */
+typedef struct dgsem_batch dgsem_batch;
+
+/* Creates the per-case handles (dgsem_create of each cases[i], all on
+ * `stream`).  On failure *out is still set so dgsem_batch_last_error can be
+ * read; release it with dgsem_batch_destroy. */
+int dgsem_batch_create(const dgsem_config* cases, int ncases, void* stream, dgsem_batch** out);
+int dgsem_batch_destroy(dgsem_batch* b);
+const char* dgsem_batch_last_error(const dgsem_batch* b);
+int dgsem_batch_sizes(const dgsem_batch* b, int* ncases, int64_t* total);
+/* Arena slice of case i and its single-case handle (owned by the batch). */
+int dgsem_batch_case(dgsem_batch* b, int i, int64_t* offset, int64_t* count, dgsem_handle** h);
+/* Per-case time step (host array of ncases doubles). */
+int dgsem_batch_set_dt(dgsem_batch* b, const double* dt);
+/* du = L(u) for every case (one launch). */
+int dgsem_batch_rhs(dgsem_batch* b, const double* u, double* du);
+/* nsteps SSPRK3 steps of every case with its own dt; u, tmp1, tmp2 are
+ * distinct arenas.  Replays the captured graph of the three stage launches;
+ * the graph is recaptured when the arenas, dt or tuning change. */
+int dgsem_batch_ssprk3_steps(dgsem_batch* b, double* u, int nsteps, double* tmp1, double* tmp2);
+/* Tuning: workgroup size (128 or 256), elements per tile for each degree
+ * (tile_elems[N], N = 0..7; 0 = as many as one workgroup holds) and the
+ * number of persistent workgroups (0 = all that fit the device at once). */
+int dgsem_batch_get_tuning(const dgsem_batch* b, int* block, int* tile_elems, int* grid);
+int dgsem_batch_set_tuning(dgsem_batch* b, int block, const int* tile_elems, int grid);
+/* Autotuner: times the batch RHS of u (written to `scratch`, an arena) over
+ * `reps` launches for each candidate workgroup size, then each degree's tile
+ * size, then the grid size, and keeps the fastest. */
+int dgsem_batch_autotune(dgsem_batch* b, const double* u, double* scratch, int reps);
+
 #ifdef __cplusplus
 }
 #endif

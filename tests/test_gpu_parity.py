@@ -14,43 +14,13 @@ import numpy as np
 import pytest
 
 import dgsem_amd as dg
-from oracle import oracle as O
+from gpu_pairs import gpu_rhs, make_pair, rel_l2
 
 pytestmark = pytest.mark.gpu
 
 RHS_TOL = 1e-12
 STEPS_TOL = 1e-10
 TWO_PI = 2.0 * math.pi
-
-
-def rel_l2(a, b):
-    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
-
-
-def make_pair(dim, N, cells, left, right, *, geometry=dg.GEOMETRY_NODAL, volume=dg.SPLIT_FORM,
-              surface=dg.FLUX_LLF, fv=dg.FLUX_LLF, amax=0.5, amin=0.001, floor=0.0):
-    oc = O.Case(dim, N, cells, left, right, volume=volume, surface_flux=surface, fv_flux=fv,
-                alpha_max=amax, alpha_min=amin, positivity_floor=floor)
-    cfg = dg.SolverConfig(dim=dim, polydeg=N, cells=cells, domain_left=left, domain_right=right,
-                          volume_integral=volume, surface_flux=surface, fv_flux=fv, alpha_max=amax,
-                          alpha_min=amin, positivity_floor=floor, geometry=geometry)
-    gs = dg.StructuredSolver(cfg)
-    if geometry == dg.GEOMETRY_AFFINE:
-        # feed the oracle the exact per-element metrics the GPU uses
-        h = [(right[k] - left[k]) / cells[k] / 2.0 for k in range(dim)]
-        Ja = np.zeros((oc.E, oc.ndofs, dim, dim))
-        for r in range(dim):
-            Ja[..., r, r] = np.prod([h[k] for k in range(dim) if k != r])
-        invJ = np.full((oc.E, oc.ndofs), 1.0 / np.prod(h))
-        oc.set_metrics(Ja, invJ)
-    return oc, gs
-
-
-def gpu_rhs(gs, u_ref):
-    u = gs.from_reference_layout(u_ref)
-    du = gs.empty()
-    gs.calc_rhs(u, du, 0.0)
-    return gs.to_reference_layout(du)
 
 
 @pytest.mark.parametrize("geometry", [dg.GEOMETRY_NODAL, dg.GEOMETRY_AFFINE])
