@@ -31,9 +31,11 @@ SOURCES = [
     ("rhs_dispatch.hip", "hipcc", "rhs_dispatch", ()),
     ("abi.hip", "hipcc", "abi", ()),
     ("batch.hip", "hipcc", "batch", ()),
+    ("adjoint.hip", "hipcc", "adjoint", ()),
 ] + [("batch_inst.hip", "hipcc", f"batch_inst_b{b}", (f"DGSEM_BATCH_BLOCK={b}",))
-     for b in (128, 256)] + [("rhs_inst.hip", "hipcc", f"rhs_inst_{d}d_n{n}", (f"DGSEM_INST_DIM={d}", f"DGSEM_INST_N={n}"))
-     for d, n in INSTANCES]
+     for b in (128, 256)] + [(f"{kind}_inst.hip", "hipcc", f"{kind}_inst_{d}d_n{n}",
+       (f"DGSEM_INST_DIM={d}", f"DGSEM_INST_N={n}"))
+     for kind in ("rhs", "adjoint") for d, n in INSTANCES]
 
 
 def _deps_newer(src: str, obj: str) -> bool:

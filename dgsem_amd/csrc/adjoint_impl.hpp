@@ -1,0 +1,712 @@
+// Tangent (JVP) and discrete adjoint (VJP) kernels of the DGSEM right-hand
+// side L(u) = -(1/J) (A(u) + R(u)): A the split-form volume integral plus the
+// LLF surface flux lifted with l_hat, R the BR1 viscous part (Navier-Stokes).
+// They differentiate exactly what rhs_kernel / gradient_kernel /
+// viscous_kernel compute, with the pointwise derivatives of physics_ad.hpp.
+//
+// Tangent: three kernels (NS) or one (Euler):
+//   jvp_grad_kernel   dg = d(grad q)             -> a.dgrad
+//   jvp_visc_kernel   dR                         -> a.dvisc
+//   jvp_rhs_kernel    dA, then -(1/J)(dA + dR) and the output combination
+// Adjoint, with r = -(1/J) w the cotangent of A + R:
+//   vjp_visc1_kernel  Rbar -> Hbar (transposed D_hat and interface lifting)
+//                     -> Fbar -> (qbar direct, gbar) -> Gbar  -> a.qbar, a.dgrad
+//   vjp_visc2_kernel  Gbar -> qbar (transposed D_hat and BR1 face averages,
+//                     gathered from the neighbours) -> ubar_visc -> a.dvisc
+//   vjp_rhs_kernel    pairs and faces -> primitive / conservative cotangents,
+//                     plus ubar_visc, and the output combination
+// Every kernel gathers what it needs from the neighbours (no atomics), so
+// results do not depend on scheduling.  The Navier-Stokes forward gradient
+// of the state must be in a.k.grad.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "parabolic_impl.hpp"
+#include "physics_ad.hpp"
+
+namespace dgsem {
+namespace detail {
+
+template <int DIM, int N>
+constexpr int ad_elems_per_block() {
+  constexpr int e = 192 / Shape<DIM, N>::LINES;
+  return e < 1 ? 1 : e;
+}
+
+__device__ __forceinline__ double combine(const AdParams& a, int64_t g, double r) {
+  double o = a.c * r;
+  if (a.b != 0.0) o += a.b * a.v[g];
+  if (a.x1) o += a.a1 * a.x1[g];
+  if (a.x2) o += a.a2 * a.x2[g];
+  return o;
+}
+
+template <int DIM, int NN>
+__device__ __forceinline__ void load_state(const double* u, int64_t e, int node, double* s) {
+  for (int v = 0; v < DIM + 2; ++v) s[v] = u[(e * (DIM + 2) + v) * NN + node];
+}
+
+template <bool NODAL, int DIM, int NN>
+__device__ __forceinline__ double sign_j(const KParams& p, int64_t e, int node) {
+  return inv_jac<NODAL, NN>(p, e, node) >= 0.0 ? 1.0 : -1.0;
+}
+
+// ---------------------------------------------------------------------------
+// Tangent
+// ---------------------------------------------------------------------------
+
+template <int DIM, int N, int EPW, bool NODAL>
+__global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) jvp_grad_kernel(const AdParams a) {
+  using S = Shape<DIM, N>;
+  constexpr int n = S::n, NV = S::NV, NN = S::NN, FN = S::FN, L = S::LINES;
+  constexpr int BLOCK = EPW * L;
+  const KParams& p = a.k;
+  __shared__ double s_dq[EPW * NV * NN];
+  __shared__ double s_g[EPW * DIM * NV * NN];
+  const int tid = threadIdx.x;
+  const int64_t e0 = (int64_t)blockIdx.x * EPW;
+  const int nvalid = (int)((p.E - e0) < EPW ? (p.E - e0) : EPW);
+
+  for (int i = tid; i < EPW * NN; i += BLOCK) {
+    const int sl = i / NN, node = i % NN;
+    if (sl >= nvalid) continue;
+    double s[NV], ds[NV], dq[NV];
+    load_state<DIM, NN>(p.u, e0 + sl, node, s);
+    load_state<DIM, NN>(a.v, e0 + sl, node, ds);
+    grad_vars_jvp<DIM>(p.gamma, s, ds, dq);
+    for (int v = 0; v < NV; ++v) s_dq[(sl * NV + v) * NN + node] = dq[v];
+  }
+  __syncthreads();
+
+  const int slot = tid / L, rem = tid % L, dir = rem / FN, line = rem % FN;
+  if (slot < nvalid) {
+    const int64_t e = e0 + slot;
+    int dofs[n];
+    for (int k = 0; k < n; ++k) dofs[k] = line_dof<DIM, N>(dir, line, k);
+    const double* dq = s_dq + slot * NV * NN;
+    double acc[n][NV];
+    for (int k = 0; k < n; ++k)
+      for (int v = 0; v < NV; ++v) {
+        double r = 0.0;
+        for (int i = 0; i < n; ++i) r += p.Dhat[k][i] * dq[v * NN + dofs[i]];
+        acc[k][v] = r;
+      }
+    for (int side = 0; side < 2; ++side) {
+      const int64_t nb = p.nbr[e * 2 * DIM + 2 * dir + side];
+      const int dof_nb = side == 0 ? dofs[n - 1] : dofs[0];
+      double s[NV], ds[NV], dqn[NV];
+      load_state<DIM, NN>(p.u, nb, dof_nb, s);
+      load_state<DIM, NN>(a.v, nb, dof_nb, ds);
+      grad_vars_jvp<DIM>(p.gamma, s, ds, dqn);
+      const int dof_own = side == 0 ? dofs[0] : dofs[n - 1];
+      const double sg = sign_j<NODAL, DIM, NN>(p, e, dof_own);
+      if (side == 0) {
+        for (int v = 0; v < NV; ++v) acc[0][v] -= sg * p.lhat0 * (0.5 * (dqn[v] + dq[v * NN + dofs[0]]));
+      } else {
+        for (int v = 0; v < NV; ++v) acc[n - 1][v] += sg * p.lhatN * (0.5 * (dq[v * NN + dofs[n - 1]] + dqn[v]));
+      }
+    }
+    for (int k = 0; k < n; ++k)
+      for (int v = 0; v < NV; ++v) s_g[((slot * DIM + dir) * NV + v) * NN + dofs[k]] = acc[k][v];
+  }
+  __syncthreads();
+  for (int i = tid; i < EPW * NN; i += BLOCK) {
+    const int sl = i / NN, node = i % NN;
+    if (sl >= nvalid) continue;
+    const int64_t e = e0 + sl;
+    const double ij = inv_jac<NODAL, NN>(p, e, node);
+    for (int c = 0; c < DIM; ++c)
+      for (int v = 0; v < NV; ++v) {
+        double g = 0.0;
+        for (int d = 0; d < DIM; ++d)
+          g += metric<NODAL, DIM, NN>(p, e, d, c, node) * s_g[((sl * DIM + d) * NV + v) * NN + node];
+        a.dgrad[((e * DIM + c) * NV + v) * NN + node] = g * ij;
+      }
+  }
+}
+
+// Tangent of Ja^d . (-F_v) at one node: out[d][v].
+template <int DIM, int NN, bool NODAL>
+__device__ __forceinline__ void contravariant_viscous_jvp(const AdParams& a, int64_t e, int node,
+                                                          double out[DIM][DIM + 2]) {
+  constexpr int NV = DIM + 2;
+  const KParams& p = a.k;
+  double s[NV], ds[NV], q[NV], dq[NV], g[DIM][NV], dg[DIM][NV], df[DIM][NV];
+  load_state<DIM, NN>(p.u, e, node, s);
+  load_state<DIM, NN>(a.v, e, node, ds);
+  grad_vars<DIM>(p.gamma, s, q);
+  grad_vars_jvp<DIM>(p.gamma, s, ds, dq);
+  for (int c = 0; c < DIM; ++c)
+    for (int v = 0; v < NV; ++v) {
+      g[c][v] = p.grad[((e * DIM + c) * NV + v) * NN + node];
+      dg[c][v] = a.dgrad[((e * DIM + c) * NV + v) * NN + node];
+    }
+  viscous_flux_jvp<DIM>(p.mu, p.kappa, q, g, dq, dg, df);
+  for (int d = 0; d < DIM; ++d)
+    for (int v = 0; v < NV; ++v) {
+      double r = 0.0;
+      for (int c = 0; c < DIM; ++c) r += metric<NODAL, DIM, NN>(p, e, d, c, node) * -df[c][v];
+      out[d][v] = r;
+    }
+}
+
+template <int DIM, int N, int EPW, bool NODAL>
+__global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) jvp_visc_kernel(const AdParams a) {
+  using S = Shape<DIM, N>;
+  constexpr int n = S::n, NV = S::NV, NN = S::NN, FN = S::FN, L = S::LINES;
+  constexpr int BLOCK = EPW * L;
+  const KParams& p = a.k;
+  __shared__ double s_f[EPW * DIM * NV * NN];
+  __shared__ double s_du[EPW * NV * NN];
+  const int tid = threadIdx.x;
+  const int64_t e0 = (int64_t)blockIdx.x * EPW;
+  const int nvalid = (int)((p.E - e0) < EPW ? (p.E - e0) : EPW);
+
+  for (int i = tid; i < EPW * NN; i += BLOCK) {
+    const int sl = i / NN, node = i % NN;
+    if (sl >= nvalid) continue;
+    double out[DIM][NV];
+    contravariant_viscous_jvp<DIM, NN, NODAL>(a, e0 + sl, node, out);
+    for (int d = 0; d < DIM; ++d)
+      for (int v = 0; v < NV; ++v) s_f[((sl * DIM + d) * NV + v) * NN + node] = out[d][v];
+  }
+  __syncthreads();
+
+  const int slot = tid / L, rem = tid % L, dir = rem / FN, line = rem % FN;
+  const bool valid = slot < nvalid;
+  double acc[n][NV];
+  int dofs[n];
+  for (int k = 0; k < n; ++k) dofs[k] = line_dof<DIM, N>(dir, line, k);
+  if (valid) {
+    const int64_t e = e0 + slot;
+    const double* f = s_f + (slot * DIM + dir) * NV * NN;
+    for (int k = 0; k < n; ++k)
+      for (int v = 0; v < NV; ++v) {
+        double r = 0.0;
+        for (int i = 0; i < n; ++i) r += p.Dhat[k][i] * f[v * NN + dofs[i]];
+        acc[k][v] = r;
+      }
+    {
+      const double sg = sign_j<NODAL, DIM, NN>(p, e, dofs[0]);
+      for (int v = 0; v < NV; ++v) acc[0][v] -= sg * f[v * NN + dofs[0]] * p.lhat0;
+    }
+    {
+      const int64_t nb = p.nbr[e * 2 * DIM + 2 * dir + 1];
+      double out[DIM][NV];
+      contravariant_viscous_jvp<DIM, NN, NODAL>(a, nb, dofs[0], out);
+      const double sg = sign_j<NODAL, DIM, NN>(p, nb, dofs[0]);
+      for (int v = 0; v < NV; ++v) acc[n - 1][v] += sg * out[dir][v] * p.lhatN;
+    }
+  }
+  for (int d = 0; d < DIM; ++d) {
+    if (valid && dir == d) {
+      for (int k = 0; k < n; ++k)
+        for (int v = 0; v < NV; ++v) {
+          const int idx = (slot * NV + v) * NN + dofs[k];
+          s_du[idx] = d == 0 ? acc[k][v] : s_du[idx] + acc[k][v];
+        }
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < EPW * NV * NN; i += BLOCK) {
+    if (i / (NV * NN) >= nvalid) continue;
+    a.dvisc[e0 * NV * NN + i] = s_du[i];
+  }
+}
+
+template <int DIM, int N, int EPW, bool NODAL, bool NS>
+__global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) jvp_rhs_kernel(const AdParams a) {
+  using S = Shape<DIM, N>;
+  constexpr int n = S::n, NV = S::NV, NN = S::NN, FN = S::FN, L = S::LINES, NP = S::NP;
+  constexpr int NQ = DIM + 3;
+  constexpr int BLOCK = EPW * L;
+  const KParams& p = a.k;
+  __shared__ double s_p[EPW * NP * NN];  // rho, v, beta, kin, p, c; later the sums
+  __shared__ double s_dq[EPW * NQ * NN];
+  double* const s_acc = s_p;
+  const int tid = threadIdx.x;
+  const int64_t e0 = (int64_t)blockIdx.x * EPW;
+  const int nvalid = (int)((p.E - e0) < EPW ? (p.E - e0) : EPW);
+
+  for (int i = tid; i < EPW * NN; i += BLOCK) {
+    const int sl = i / NN, node = i % NN;
+    if (sl >= nvalid) continue;
+    double s[NV], ds[NV];
+    load_state<DIM, NN>(p.u, e0 + sl, node, s);
+    load_state<DIM, NN>(a.v, e0 + sl, node, ds);
+    const NodePrims<DIM> q = node_prims<DIM>(p.gamma, s);
+    PrimTangent<DIM> dq;
+    prims_jvp<DIM>(p.gamma, s, q, ds, dq);
+    double* d = s_p + sl * NP * NN + node;
+    d[0] = q.rho;
+    for (int k = 0; k < DIM; ++k) d[(1 + k) * NN] = q.v[k];
+    d[(DIM + 1) * NN] = q.beta;
+    d[(DIM + 2) * NN] = q.kin;
+    d[(DIM + 3) * NN] = q.p;
+    d[(DIM + 4) * NN] = q.c;
+    double* t = s_dq + sl * NQ * NN + node;
+    t[0] = dq.rho;
+    for (int k = 0; k < DIM; ++k) t[(1 + k) * NN] = dq.v[k];
+    t[(DIM + 1) * NN] = dq.beta;
+    t[(DIM + 2) * NN] = dq.kin;
+  }
+  __syncthreads();
+
+  const int slot = tid / L, rem = tid % L, dir = rem / FN, line = rem % FN;
+  const bool valid = slot < nvalid;
+  int dofs[n];
+  for (int k = 0; k < n; ++k) dofs[k] = line_dof<DIM, N>(dir, line, k);
+  double acc[n][NV];
+  for (int k = 0; k < n; ++k)
+    for (int v = 0; v < NV; ++v) acc[k][v] = 0.0;
+  if (valid) {
+    const int64_t e = e0 + slot;
+    const double* pr = s_p + slot * NP * NN;
+    const double* tq = s_dq + slot * NQ * NN;
+    auto prims = [&](int dof) {
+      NodePrims<DIM> q;
+      q.rho = pr[dof];
+      for (int c = 0; c < DIM; ++c) q.v[c] = pr[(1 + c) * NN + dof];
+      q.beta = pr[(DIM + 1) * NN + dof];
+      q.kin = pr[(DIM + 2) * NN + dof];
+      q.p = pr[(DIM + 3) * NN + dof];
+      q.c = pr[(DIM + 4) * NN + dof];
+      return q;
+    };
+    auto tangent = [&](int dof) {
+      PrimTangent<DIM> t;
+      t.rho = tq[dof];
+      for (int c = 0; c < DIM; ++c) t.v[c] = tq[(1 + c) * NN + dof];
+      t.beta = tq[(DIM + 1) * NN + dof];
+      t.kin = tq[(DIM + 2) * NN + dof];
+      return t;
+    };
+    auto ja = [&](int64_t el, int dof, double* out) {
+      for (int c = 0; c < DIM; ++c) out[c] = metric<NODAL, DIM, NN>(p, el, dir, c, dof);
+    };
+    for (int ia = 0; ia < n - 1; ++ia) {
+      const NodePrims<DIM> qa = prims(dofs[ia]);
+      const PrimTangent<DIM> ta = tangent(dofs[ia]);
+      double jaa[DIM];
+      ja(e, dofs[ia], jaa);
+      for (int ib = ia + 1; ib < n; ++ib) {
+        const NodePrims<DIM> qb = prims(dofs[ib]);
+        const PrimTangent<DIM> tb = tangent(dofs[ib]);
+        double jab[DIM], nrm[DIM], df[NV];
+        ja(e, dofs[ib], jab);
+        for (int c = 0; c < DIM; ++c) nrm[c] = NODAL ? 0.5 * (jaa[c] + jab[c]) : jaa[c];
+        chandrashekar_jvp<DIM>(p.inv_gm1, qa, qb, ta, tb, nrm, df);
+        const double wa = p.Ds[ia][ib], wb = p.Ds[ib][ia];
+        for (int v = 0; v < NV; ++v) {
+          acc[ia][v] += wa * df[v];
+          acc[ib][v] += wb * df[v];
+        }
+      }
+    }
+    // LLF faces, both states and their tangents from memory
+    for (int side = 0; side < 2; ++side) {
+      const bool lower = side == 0;
+      const int64_t nb = p.nbr[e * 2 * DIM + 2 * dir + side];
+      double ul[NV], ur[NV], dul[NV], dur[NV], nrm[DIM], df[NV];
+      // the face's lower element contributes its last node, the upper one its first
+      const int64_t el = lower ? nb : e, er = lower ? e : nb;
+      const int dl = dofs[n - 1], dr = dofs[0];
+      load_state<DIM, NN>(p.u, el, dl, ul);
+      load_state<DIM, NN>(p.u, er, dr, ur);
+      load_state<DIM, NN>(a.v, el, dl, dul);
+      load_state<DIM, NN>(a.v, er, dr, dur);
+      // the normal of the face's upper element at its first node
+      ja(er, dr, nrm);
+      const double sg = sign_j<NODAL, DIM, NN>(p, er, dr);
+      for (int c = 0; c < DIM; ++c) nrm[c] *= sg;
+      llf_jvp<DIM>(p.gamma, ul, node_prims<DIM>(p.gamma, ul), ur, node_prims<DIM>(p.gamma, ur), dul,
+                   dur, nrm, df);
+      if (lower) {
+        for (int v = 0; v < NV; ++v) acc[0][v] -= df[v] * p.lhat0;
+      } else {
+        for (int v = 0; v < NV; ++v) acc[n - 1][v] += df[v] * p.lhatN;
+      }
+    }
+  }
+  __syncthreads();  // s_acc aliases the primitives
+  for (int d = 0; d < DIM; ++d) {
+    if (valid && dir == d) {
+      for (int k = 0; k < n; ++k)
+        for (int v = 0; v < NV; ++v) {
+          const int idx = (slot * NV + v) * NN + dofs[k];
+          s_acc[idx] = d == 0 ? acc[k][v] : s_acc[idx] + acc[k][v];
+        }
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < EPW * NN; i += BLOCK) {
+    const int sl = i / NN, node = i % NN;
+    if (sl >= nvalid) continue;
+    const int64_t e = e0 + sl;
+    const double fac = -inv_jac<NODAL, NN>(p, e, node);
+    for (int v = 0; v < NV; ++v) {
+      const int64_t g = (e * NV + v) * NN + node;
+      double r = s_acc[(sl * NV + v) * NN + node];
+      if constexpr (NS) r += a.dvisc[g];
+      a.out[g] = combine(a, g, r * fac);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint
+// ---------------------------------------------------------------------------
+
+// r = -(1/J) w of element e at node: the cotangent of A + R.
+template <int DIM, int NN, bool NODAL>
+__device__ __forceinline__ void load_r(const AdParams& a, int64_t e, int node, double* r) {
+  const double f = -inv_jac<NODAL, NN>(a.k, e, node);
+  for (int v = 0; v < DIM + 2; ++v) r[v] = f * a.v[(e * (DIM + 2) + v) * NN + node];
+}
+
+template <int DIM, int N, int EPW, bool NODAL>
+__global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_visc1_kernel(const AdParams a) {
+  using S = Shape<DIM, N>;
+  constexpr int n = S::n, NV = S::NV, NN = S::NN, FN = S::FN, L = S::LINES;
+  constexpr int BLOCK = EPW * L;
+  const KParams& p = a.k;
+  __shared__ double s_r[EPW * NV * NN];
+  __shared__ double s_hb[EPW * DIM * NV * NN];
+  const int tid = threadIdx.x;
+  const int64_t e0 = (int64_t)blockIdx.x * EPW;
+  const int nvalid = (int)((p.E - e0) < EPW ? (p.E - e0) : EPW);
+
+  for (int i = tid; i < EPW * NN; i += BLOCK) {
+    const int sl = i / NN, node = i % NN;
+    if (sl >= nvalid) continue;
+    double r[NV];
+    load_r<DIM, NN, NODAL>(a, e0 + sl, node, r);
+    for (int v = 0; v < NV; ++v) s_r[(sl * NV + v) * NN + node] = r[v];
+  }
+  __syncthreads();
+
+  const int slot = tid / L, rem = tid % L, dir = rem / FN, line = rem % FN;
+  if (slot < nvalid) {
+    const int64_t e = e0 + slot;
+    int dofs[n];
+    for (int k = 0; k < n; ++k) dofs[k] = line_dof<DIM, N>(dir, line, k);
+    const double* r = s_r + slot * NV * NN;
+    // Hbar_k = sum_k' Dhat[k'][k] r_k'  (transpose of the weak divergence)
+    double hb[n][NV];
+    for (int k = 0; k < n; ++k)
+      for (int v = 0; v < NV; ++v) {
+        double h = 0.0;
+        for (int i = 0; i < n; ++i) h += p.Dhat[i][k] * r[v * NN + dofs[i]];
+        hb[k][v] = h;
+      }
+    // the first node's value is lifted by this element (-l_hat0, lower face)
+    // and by the lower neighbour (+l_hatN, its upper face, whose primary
+    // element this is)
+    {
+      const int64_t lo = p.nbr[e * 2 * DIM + 2 * dir];
+      double rl[NV];
+      load_r<DIM, NN, NODAL>(a, lo, dofs[n - 1], rl);
+      const double sg = sign_j<NODAL, DIM, NN>(p, e, dofs[0]);
+      for (int v = 0; v < NV; ++v) hb[0][v] += sg * (p.lhatN * rl[v] - p.lhat0 * r[v * NN + dofs[0]]);
+    }
+    for (int k = 0; k < n; ++k)
+      for (int v = 0; v < NV; ++v) s_hb[((slot * DIM + dir) * NV + v) * NN + dofs[k]] = hb[k][v];
+  }
+  __syncthreads();
+  for (int i = tid; i < EPW * NN; i += BLOCK) {
+    const int sl = i / NN, node = i % NN;
+    if (sl >= nvalid) continue;
+    const int64_t e = e0 + sl;
+    double s[NV], q[NV], g[DIM][NV], fb[DIM][NV], qb[NV], gb[DIM][NV];
+    load_state<DIM, NN>(p.u, e, node, s);
+    grad_vars<DIM>(p.gamma, s, q);
+    for (int c = 0; c < DIM; ++c)
+      for (int v = 0; v < NV; ++v) {
+        g[c][v] = p.grad[((e * DIM + c) * NV + v) * NN + node];
+        gb[c][v] = 0.0;
+        double f = 0.0;
+        for (int d = 0; d < DIM; ++d)
+          f -= metric<NODAL, DIM, NN>(p, e, d, c, node) * s_hb[((sl * DIM + d) * NV + v) * NN + node];
+        fb[c][v] = f;
+      }
+    for (int v = 0; v < NV; ++v) qb[v] = 0.0;
+    viscous_flux_vjp<DIM>(p.mu, p.kappa, q, g, fb, qb, gb);
+    for (int v = 0; v < NV; ++v) a.qbar[(e * NV + v) * NN + node] = qb[v];
+    const double ij = inv_jac<NODAL, NN>(p, e, node);
+    for (int d = 0; d < DIM; ++d)
+      for (int v = 0; v < NV; ++v) {
+        double G = 0.0;
+        for (int c = 0; c < DIM; ++c) G += metric<NODAL, DIM, NN>(p, e, d, c, node) * gb[c][v];
+        a.dgrad[((e * DIM + d) * NV + v) * NN + node] = ij * G;
+      }
+  }
+}
+
+template <int DIM, int N, int EPW, bool NODAL>
+__global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_visc2_kernel(const AdParams a) {
+  using S = Shape<DIM, N>;
+  constexpr int n = S::n, NV = S::NV, NN = S::NN, FN = S::FN, L = S::LINES;
+  constexpr int BLOCK = EPW * L;
+  const KParams& p = a.k;
+  __shared__ double s_gb[EPW * DIM * NV * NN];
+  __shared__ double s_qb[EPW * NV * NN];
+  const int tid = threadIdx.x;
+  const int64_t e0 = (int64_t)blockIdx.x * EPW;
+  const int nvalid = (int)((p.E - e0) < EPW ? (p.E - e0) : EPW);
+
+  for (int i = tid; i < nvalid * DIM * NV * NN; i += BLOCK) s_gb[i] = a.dgrad[e0 * DIM * NV * NN + i];
+  __syncthreads();
+
+  const int slot = tid / L, rem = tid % L, dir = rem / FN, line = rem % FN;
+  const bool valid = slot < nvalid;
+  int dofs[n];
+  for (int k = 0; k < n; ++k) dofs[k] = line_dof<DIM, N>(dir, line, k);
+  double qb[n][NV];
+  if (valid) {
+    const int64_t e = e0 + slot;
+    const double* G = s_gb + (slot * DIM + dir) * NV * NN;
+    for (int k = 0; k < n; ++k)
+      for (int v = 0; v < NV; ++v) {
+        double h = 0.0;
+        for (int i = 0; i < n; ++i) h += p.Dhat[i][k] * G[v * NN + dofs[i]];
+        qb[k][v] = h;
+      }
+    // BR1 averages q_hat = (q_L + q_R)/2 on this element's faces and on the
+    // neighbours' faces that used this element's face values
+    const int64_t lo = p.nbr[e * 2 * DIM + 2 * dir], hi = p.nbr[e * 2 * DIM + 2 * dir + 1];
+    const double sg0 = sign_j<NODAL, DIM, NN>(p, e, dofs[0]);
+    const double sgn = sign_j<NODAL, DIM, NN>(p, e, dofs[n - 1]);
+    const double sg_lo = sign_j<NODAL, DIM, NN>(p, lo, dofs[n - 1]);
+    const double sg_hi = sign_j<NODAL, DIM, NN>(p, hi, dofs[0]);
+    for (int v = 0; v < NV; ++v) {
+      const double g_lo = a.dgrad[((lo * DIM + dir) * NV + v) * NN + dofs[n - 1]];
+      const double g_hi = a.dgrad[((hi * DIM + dir) * NV + v) * NN + dofs[0]];
+      qb[0][v] += 0.5 * (sg_lo * p.lhatN * g_lo - sg0 * p.lhat0 * G[v * NN + dofs[0]]);
+      qb[n - 1][v] += 0.5 * (sgn * p.lhatN * G[v * NN + dofs[n - 1]] - sg_hi * p.lhat0 * g_hi);
+    }
+  }
+  for (int d = 0; d < DIM; ++d) {
+    if (valid && dir == d) {
+      for (int k = 0; k < n; ++k)
+        for (int v = 0; v < NV; ++v) {
+          const int idx = (slot * NV + v) * NN + dofs[k];
+          s_qb[idx] = d == 0 ? qb[k][v] : s_qb[idx] + qb[k][v];
+        }
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < EPW * NN; i += BLOCK) {
+    const int sl = i / NN, node = i % NN;
+    if (sl >= nvalid) continue;
+    const int64_t e = e0 + sl;
+    double s[NV], q[NV], ub[NV];
+    load_state<DIM, NN>(p.u, e, node, s);
+    for (int v = 0; v < NV; ++v) {
+      q[v] = s_qb[(sl * NV + v) * NN + node] + a.qbar[(e * NV + v) * NN + node];
+      ub[v] = 0.0;
+    }
+    grad_vars_vjp<DIM>(p.gamma, s, q, ub);
+    for (int v = 0; v < NV; ++v) a.dvisc[(e * NV + v) * NN + node] = ub[v];
+  }
+}
+
+template <int DIM, int N, int EPW, bool NODAL, bool NS>
+__global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(const AdParams a) {
+  using S = Shape<DIM, N>;
+  constexpr int n = S::n, NV = S::NV, NN = S::NN, FN = S::FN, L = S::LINES, NP = S::NP;
+  constexpr int NQ = DIM + 3;
+  constexpr int BLOCK = EPW * L;
+  const KParams& p = a.k;
+  __shared__ double s_p[EPW * NP * NN];  // rho, v, beta, kin, p, c
+  __shared__ double s_r[EPW * NV * NN];
+  __shared__ double s_qb[EPW * NQ * NN];
+  __shared__ double s_ub[EPW * NV * NN];
+  const int tid = threadIdx.x;
+  const int64_t e0 = (int64_t)blockIdx.x * EPW;
+  const int nvalid = (int)((p.E - e0) < EPW ? (p.E - e0) : EPW);
+
+  for (int i = tid; i < EPW * NN; i += BLOCK) {
+    const int sl = i / NN, node = i % NN;
+    for (int v = 0; v < NV; ++v) s_ub[(sl * NV + v) * NN + node] = 0.0;
+    if (sl >= nvalid) continue;
+    double s[NV], r[NV];
+    load_state<DIM, NN>(p.u, e0 + sl, node, s);
+    const NodePrims<DIM> q = node_prims<DIM>(p.gamma, s);
+    double* d = s_p + sl * NP * NN + node;
+    d[0] = q.rho;
+    for (int k = 0; k < DIM; ++k) d[(1 + k) * NN] = q.v[k];
+    d[(DIM + 1) * NN] = q.beta;
+    d[(DIM + 2) * NN] = q.kin;
+    d[(DIM + 3) * NN] = q.p;
+    d[(DIM + 4) * NN] = q.c;
+    load_r<DIM, NN, NODAL>(a, e0 + sl, node, r);
+    for (int v = 0; v < NV; ++v) s_r[(sl * NV + v) * NN + node] = r[v];
+  }
+  __syncthreads();
+
+  const int slot = tid / L, rem = tid % L, dir = rem / FN, line = rem % FN;
+  const bool valid = slot < nvalid;
+  int dofs[n];
+  for (int k = 0; k < n; ++k) dofs[k] = line_dof<DIM, N>(dir, line, k);
+  PrimTangent<DIM> qb[n];
+  for (int k = 0; k < n; ++k) {
+    qb[k].rho = qb[k].beta = qb[k].kin = 0.0;
+    for (int c = 0; c < DIM; ++c) qb[k].v[c] = 0.0;
+  }
+  double ub_lo[NV], ub_hi[NV];
+  for (int v = 0; v < NV; ++v) ub_lo[v] = ub_hi[v] = 0.0;
+  if (valid) {
+    const int64_t e = e0 + slot;
+    const double* pr = s_p + slot * NP * NN;
+    const double* rr = s_r + slot * NV * NN;
+    auto prims = [&](int dof) {
+      NodePrims<DIM> q;
+      q.rho = pr[dof];
+      for (int c = 0; c < DIM; ++c) q.v[c] = pr[(1 + c) * NN + dof];
+      q.beta = pr[(DIM + 1) * NN + dof];
+      q.kin = pr[(DIM + 2) * NN + dof];
+      q.p = pr[(DIM + 3) * NN + dof];
+      q.c = pr[(DIM + 4) * NN + dof];
+      return q;
+    };
+    auto ja = [&](int64_t el, int dof, double* out) {
+      for (int c = 0; c < DIM; ++c) out[c] = metric<NODAL, DIM, NN>(p, el, dir, c, dof);
+    };
+    for (int ia = 0; ia < n - 1; ++ia) {
+      const NodePrims<DIM> qa = prims(dofs[ia]);
+      double jaa[DIM];
+      ja(e, dofs[ia], jaa);
+      for (int ib = ia + 1; ib < n; ++ib) {
+        const NodePrims<DIM> qbn = prims(dofs[ib]);
+        double jab[DIM], nrm[DIM], g[NV];
+        ja(e, dofs[ib], jab);
+        for (int c = 0; c < DIM; ++c) nrm[c] = NODAL ? 0.5 * (jaa[c] + jab[c]) : jaa[c];
+        const double wa = p.Ds[ia][ib], wb = p.Ds[ib][ia];
+        for (int v = 0; v < NV; ++v) g[v] = wa * rr[v * NN + dofs[ia]] + wb * rr[v * NN + dofs[ib]];
+        chandrashekar_vjp<DIM>(p.inv_gm1, qa, qbn, nrm, g, qb[ia], qb[ib]);
+      }
+    }
+    // LLF faces: this element's share of each face flux's adjoint
+    for (int side = 0; side < 2; ++side) {
+      const bool lower = side == 0;
+      const int64_t nb = p.nbr[e * 2 * DIM + 2 * dir + side];
+      const int64_t el = lower ? nb : e, er = lower ? e : nb;
+      const int dl = dofs[n - 1], dr = dofs[0];
+      double ul[NV], ur[NV], nrm[DIM], g[NV], rl[NV], rrv[NV];
+      load_state<DIM, NN>(p.u, el, dl, ul);
+      load_state<DIM, NN>(p.u, er, dr, ur);
+      load_r<DIM, NN, NODAL>(a, el, dl, rl);
+      load_r<DIM, NN, NODAL>(a, er, dr, rrv);
+      ja(er, dr, nrm);
+      const double sg = sign_j<NODAL, DIM, NN>(p, er, dr);
+      for (int c = 0; c < DIM; ++c) nrm[c] *= sg;
+      // the face flux enters the upper element's first node with -l_hat0
+      // and the lower element's last node with +l_hatN
+      for (int v = 0; v < NV; ++v) g[v] = p.lhatN * rl[v] - p.lhat0 * rrv[v];
+      llf_vjp_side<DIM>(p.gamma, ul, node_prims<DIM>(p.gamma, ul), ur, node_prims<DIM>(p.gamma, ur),
+                        nrm, g, !lower, lower ? ub_lo : ub_hi);
+    }
+  }
+  for (int d = 0; d < DIM; ++d) {
+    if (valid && dir == d) {
+      for (int k = 0; k < n; ++k) {
+        double* t = s_qb + slot * NQ * NN + dofs[k];
+        if (d == 0) {
+          t[0] = qb[k].rho;
+          for (int c = 0; c < DIM; ++c) t[(1 + c) * NN] = qb[k].v[c];
+          t[(DIM + 1) * NN] = qb[k].beta;
+          t[(DIM + 2) * NN] = qb[k].kin;
+        } else {
+          t[0] += qb[k].rho;
+          for (int c = 0; c < DIM; ++c) t[(1 + c) * NN] += qb[k].v[c];
+          t[(DIM + 1) * NN] += qb[k].beta;
+          t[(DIM + 2) * NN] += qb[k].kin;
+        }
+      }
+      for (int v = 0; v < NV; ++v) {
+        s_ub[(slot * NV + v) * NN + dofs[0]] += ub_lo[v];
+        s_ub[(slot * NV + v) * NN + dofs[n - 1]] += ub_hi[v];
+      }
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < EPW * NN; i += BLOCK) {
+    const int sl = i / NN, node = i % NN;
+    if (sl >= nvalid) continue;
+    const int64_t e = e0 + sl;
+    const double* pr = s_p + sl * NP * NN + node;
+    NodePrims<DIM> q;
+    q.rho = pr[0];
+    for (int c = 0; c < DIM; ++c) q.v[c] = pr[(1 + c) * NN];
+    q.beta = pr[(DIM + 1) * NN];
+    q.kin = pr[(DIM + 2) * NN];
+    q.p = pr[(DIM + 3) * NN];
+    q.c = pr[(DIM + 4) * NN];
+    const double* t = s_qb + sl * NQ * NN + node;
+    PrimTangent<DIM> qbar;
+    qbar.rho = t[0];
+    for (int c = 0; c < DIM; ++c) qbar.v[c] = t[(1 + c) * NN];
+    qbar.beta = t[(DIM + 1) * NN];
+    qbar.kin = t[(DIM + 2) * NN];
+    double ub[NV];
+    for (int v = 0; v < NV; ++v) ub[v] = s_ub[(sl * NV + v) * NN + node];
+    prims_vjp<DIM>(p.gamma, q, qbar, 0.0, ub);
+    for (int v = 0; v < NV; ++v) {
+      const int64_t g = (e * NV + v) * NN + node;
+      double r = ub[v];
+      if constexpr (NS) r += a.dvisc[g];
+      a.out[g] = combine(a, g, r);
+    }
+  }
+}
+
+template <int DIM, int N, bool NODAL>
+hipError_t launch_jvp_t(bool ns, const AdParams& a, hipStream_t s) {
+  constexpr int EPW = ad_elems_per_block<DIM, N>();
+  constexpr int BLOCK = EPW * Shape<DIM, N>::LINES;
+  const int64_t nb = (a.k.E + EPW - 1) / EPW;
+  if (nb <= 0) return hipSuccess;
+  if (ns) {
+    hipLaunchKernelGGL((jvp_grad_kernel<DIM, N, EPW, NODAL>), dim3((unsigned)nb), dim3(BLOCK), 0, s, a);
+    hipLaunchKernelGGL((jvp_visc_kernel<DIM, N, EPW, NODAL>), dim3((unsigned)nb), dim3(BLOCK), 0, s, a);
+    hipLaunchKernelGGL((jvp_rhs_kernel<DIM, N, EPW, NODAL, true>), dim3((unsigned)nb), dim3(BLOCK), 0,
+                       s, a);
+  } else {
+    hipLaunchKernelGGL((jvp_rhs_kernel<DIM, N, EPW, NODAL, false>), dim3((unsigned)nb), dim3(BLOCK), 0,
+                       s, a);
+  }
+  return hipGetLastError();
+}
+
+template <int DIM, int N, bool NODAL>
+hipError_t launch_vjp_t(bool ns, const AdParams& a, hipStream_t s) {
+  constexpr int EPW = ad_elems_per_block<DIM, N>();
+  constexpr int BLOCK = EPW * Shape<DIM, N>::LINES;
+  const int64_t nb = (a.k.E + EPW - 1) / EPW;
+  if (nb <= 0) return hipSuccess;
+  if (ns) {
+    hipLaunchKernelGGL((vjp_visc1_kernel<DIM, N, EPW, NODAL>), dim3((unsigned)nb), dim3(BLOCK), 0, s, a);
+    hipLaunchKernelGGL((vjp_visc2_kernel<DIM, N, EPW, NODAL>), dim3((unsigned)nb), dim3(BLOCK), 0, s, a);
+    hipLaunchKernelGGL((vjp_rhs_kernel<DIM, N, EPW, NODAL, true>), dim3((unsigned)nb), dim3(BLOCK), 0,
+                       s, a);
+  } else {
+    hipLaunchKernelGGL((vjp_rhs_kernel<DIM, N, EPW, NODAL, false>), dim3((unsigned)nb), dim3(BLOCK), 0,
+                       s, a);
+  }
+  return hipGetLastError();
+}
+
+template <int DIM, int N, bool NODAL>
+hipError_t launch_gradient_t(const KParams& p, hipStream_t s) {
+  constexpr int EPW = parabolic_elems_per_block<DIM, N>();
+  constexpr int BLOCK = EPW * Shape<DIM, N>::LINES;
+  const int64_t nb = (p.E + EPW - 1) / EPW;
+  if (nb <= 0) return hipSuccess;
+  hipLaunchKernelGGL((gradient_kernel<DIM, N, EPW, NODAL>), dim3((unsigned)nb), dim3(BLOCK), 0, s, p);
+  return hipGetLastError();
+}
+
+}  // namespace detail
+}  // namespace dgsem

@@ -28,6 +28,15 @@ struct dgsem_handle {
   double* d_grad = nullptr;  // Navier-Stokes gradient [E][dim][var][dof]
   double* d_visc = nullptr;  // Navier-Stokes viscous RHS part [E][var][dof]
   dgsem::KParams base{};
+  // tangent / adjoint scratch, allocated on first use: d_ad_dgrad
+  // [E][dim][NV][NN], d_ad_dvisc and d_ad_qbar [E][NV][NN] (Navier-Stokes),
+  // d_wq the tensor-product quadrature weight of each node, d_ke_partial the
+  // per-block sums of the kinetic-energy reduction
+  double* d_ad_dgrad = nullptr;
+  double* d_ad_dvisc = nullptr;
+  double* d_ad_qbar = nullptr;
+  double* d_wq = nullptr;
+  double* d_ke_partial = nullptr;
   std::string err;
 };
 

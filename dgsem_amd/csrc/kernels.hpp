@@ -70,6 +70,51 @@ hipError_t launch_pack(const double* src, double* dst, int64_t E, int nn, int nv
 hipError_t launch_pack_left(const double* src, double* dst, const int cells[3], int nn, int nv,
                             bool to_internal, hipStream_t stream);
 
+// Tangent (JVP) and adjoint (VJP) of the right-hand side at the state k.u
+// (adjoint_impl.hpp).  Op is J = dL/du (tangent) or J^T (adjoint) and the
+// kernels write
+//     out = a1 x1 + a2 x2 + b v + c Op(v)
+// (x1, x2 may be null), which covers the tangent and adjoint of every SSPRK3
+// stage.  Navier-Stokes needs k.grad = the forward gradient of k.u and the
+// scratch arrays dgrad [E][dim][NV][NN], dvisc and qbar [E][NV][NN].
+struct AdParams {
+  KParams k;
+  const double* v;  // tangent direction or cotangent
+  const double* x1;
+  const double* x2;
+  double a1, a2, b, c;
+  double* out;
+  double* dgrad;
+  double* dvisc;
+  double* qbar;
+};
+
+template <int DIM, int N>
+hipError_t launch_jvp_dn(bool nodal, const AdParams& a, hipStream_t stream);
+template <int DIM, int N>
+hipError_t launch_vjp_dn(bool nodal, const AdParams& a, hipStream_t stream);
+// Launch the tangent or adjoint kernels for (dim, N); the Navier-Stokes
+// forward gradient must already be in a.k.grad.
+hipError_t launch_jvp(int dim, int N, bool nodal, bool navier_stokes, const AdParams& a,
+                      hipStream_t stream);
+hipError_t launch_vjp(int dim, int N, bool nodal, bool navier_stokes, const AdParams& a,
+                      hipStream_t stream);
+// Forward BR1 gradient alone (a.k.grad_out from a.k.u).
+template <int DIM, int N>
+hipError_t launch_gradient_dn(bool nodal, const KParams& p, hipStream_t stream);
+hipError_t launch_gradient(int dim, int N, bool nodal, const KParams& p, hipStream_t stream);
+
+// Kinetic energy sum_e sum_i w_i / |1/J_i| |m_i|^2 / (2 rho_i) into *result
+// (device), and its gradient: grad += scale * dKE/du.  partial: scratch of
+// ke_partials(E) doubles.
+int64_t ke_partials(int64_t E);
+hipError_t launch_kinetic_energy(int dim, int nn, const double* w_nodes, bool nodal,
+                                 const KParams& p, double* partial, double* result,
+                                 hipStream_t stream);
+hipError_t launch_kinetic_energy_grad(int dim, int nn, const double* w_nodes, bool nodal,
+                                      const KParams& p, double scale, double* grad,
+                                      hipStream_t stream);
+
 // Batched ensembles of 2D cases (batch_impl.hpp).  A tile is up to
 // batch_slots(block, N) consecutive elements of one case.
 struct BatchTile {

@@ -10,15 +10,24 @@
 // src/base/numerical_flux_detail.hpp:9-21.
 #pragma once
 
-#include <hip/hip_runtime.h>
-
 #include <cmath>
+
+// The functions are used by the HIP kernels and, compiled by a host C++
+// compiler, by the CPU unit tests of their derivatives.
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define DG_HD __host__ __device__ __forceinline__
+#else
+#define DG_HD inline
+#endif
 
 namespace dgsem {
 
-enum FluxKind : int { kFluxLLF = 0, kFluxHLLC = 1 };
+using std::fabs;
+using std::log;
+using std::sqrt;
 
-#define DG_HD __host__ __device__ __forceinline__
+enum FluxKind : int { kFluxLLF = 0, kFluxHLLC = 1 };
 
 DG_HD double dmax(double a, double b) { return (a < b) ? b : a; }
 DG_HD double dmin(double a, double b) { return (b < a) ? b : a; }

@@ -1,0 +1,468 @@
+// Tangent (JVP) and adjoint (VJP) of the pointwise physics in physics.hpp
+// and parabolic_impl.hpp, derived by hand from the implemented formulas.
+//
+// Every nonlinear map y = F(x) here has a pair of functions:
+//   *_jvp: dy = F'(x) dx
+//   *_vjp: xbar += F'(x)^T ybar   (accumulates)
+// built from the same partial derivatives, so <ybar, F' dx> = <F'^T ybar, dx>
+// holds to rounding.  Branches (the series / logarithm forms of the
+// logarithmic mean, the max in the LLF wave speed) follow the forward
+// evaluation's choice at x: these are the derivatives of the function the
+// forward kernels compute, i.e. a discrete tangent and adjoint.
+//
+// The blending factor of shock capturing is not differentiated (it is frozen
+// at its forward value by the callers).
+#pragma once
+
+#include "physics.hpp"
+
+namespace dgsem {
+
+// Node primitives used by the two-point flux: q = (rho, v[DIM], beta, kin),
+// beta = rho / (2 p), kin = |v|^2 / 2, p = (gamma - 1)(E - rho kin).
+template <int DIM>
+struct PrimTangent {
+  double rho, v[DIM], beta, kin;
+};
+
+template <int DIM>
+DG_HD void prims_jvp(double gamma, const double* u, const NodePrims<DIM>& q, const double* du,
+                     PrimTangent<DIM>& dq) {
+  const double ir = 1.0 / q.rho;
+  dq.rho = du[0];
+  double dkin = 0.0;
+  for (int k = 0; k < DIM; ++k) {
+    dq.v[k] = (du[k + 1] - q.v[k] * du[0]) * ir;
+    dkin += q.v[k] * dq.v[k];
+  }
+  dq.kin = dkin;
+  const double dp = (gamma - 1.0) * (du[DIM + 1] - q.kin * du[0] - q.rho * dkin);
+  dq.beta = q.beta * (du[0] * ir - dp / q.p);
+  (void)u;
+}
+
+// ubar += (dq/du)^T qbar; pbar is an extra cotangent of the pressure.
+template <int DIM>
+DG_HD void prims_vjp(double gamma, const NodePrims<DIM>& q, const PrimTangent<DIM>& qbar,
+                     double pbar, double* ubar) {
+  const double ir = 1.0 / q.rho;
+  double rb = qbar.rho + qbar.beta * q.beta * ir;
+  double pb = pbar - qbar.beta * q.beta / q.p;
+  double vb[DIM];
+  for (int k = 0; k < DIM; ++k) vb[k] = qbar.v[k] + qbar.kin * q.v[k];
+  // p = (gamma - 1) (E - rho kin(v)), with v held as an independent variable
+  ubar[DIM + 1] += (gamma - 1.0) * pb;
+  rb -= (gamma - 1.0) * q.kin * pb;
+  for (int k = 0; k < DIM; ++k) vb[k] -= (gamma - 1.0) * q.rho * q.v[k] * pb;
+  // v = m / rho
+  for (int k = 0; k < DIM; ++k) {
+    ubar[k + 1] += vb[k] * ir;
+    rb -= vb[k] * q.v[k] * ir;
+  }
+  ubar[0] += rb;
+}
+
+// Logarithmic mean and its partial derivatives.  `series` selects the
+// branch the forward evaluation took.
+DG_HD void ln_mean_partials(double x, double y, bool series, double& L, double& Lx, double& Ly) {
+  if (series) {
+    const double s = x + y, d = x - y;
+    const double is = 1.0 / s;
+    const double f = d * d * is * is;
+    const double P = 2.0 + f * (2.0 / 3.0 + f * (2.0 / 5.0 + f * (2.0 / 7.0)));
+    const double dP = 2.0 / 3.0 + f * (4.0 / 5.0 + f * (6.0 / 7.0));
+    const double fx = 2.0 * d * is * is - 2.0 * d * d * is * is * is;
+    const double fy = -2.0 * d * is * is - 2.0 * d * d * is * is * is;
+    L = s / P;
+    const double c = s * dP / (P * P);
+    Lx = 1.0 / P - c * fx;
+    Ly = 1.0 / P - c * fy;
+  } else {
+    const double D = log(y / x);
+    L = (y - x) / D;
+    Lx = (L / x - 1.0) / D;
+    Ly = (1.0 - L / y) / D;
+  }
+}
+
+// Means of the Chandrashekar flux and their partial derivatives with
+// respect to (rho_a, rho_b) and (beta_a, beta_b): rho_mean, its inverse
+// beta mean B = 1 / ln_mean(beta_a, beta_b) and p_mean = (rho_a + rho_b) /
+// (2 (beta_a + beta_b)).  The branch test is the forward's.
+struct ChandrashekarMeans {
+  double rho, rho_a, rho_b;  // rho_mean and partials
+  double B, B_a, B_b;        // inverse beta mean and partials
+  double p, p_rho, p_beta;   // p_mean, d/d(rho_a or rho_b), d/d(beta_a or beta_b)
+};
+
+template <int DIM>
+DG_HD ChandrashekarMeans chandrashekar_means(const NodePrims<DIM>& a, const NodePrims<DIM>& b) {
+  const double sr = a.rho + b.rho, sb = a.beta + b.beta;
+  const double nr = a.rho * (a.rho - 2.0 * b.rho) + b.rho * b.rho;
+  const double dr = a.rho * (a.rho + 2.0 * b.rho) + b.rho * b.rho;
+  const double nb = a.beta * (a.beta - 2.0 * b.beta) + b.beta * b.beta;
+  const double db = a.beta * (a.beta + 2.0 * b.beta) + b.beta * b.beta;
+  const double r1 = 1.0 / (dr * db);
+  const double f2r = nr * (db * r1), f2b = nb * (dr * r1);
+  ChandrashekarMeans m;
+  ln_mean_partials(a.rho, b.rho, f2r < 1.0e-4, m.rho, m.rho_a, m.rho_b);
+  double Lb, Lba, Lbb;
+  ln_mean_partials(a.beta, b.beta, f2b < 1.0e-4, Lb, Lba, Lbb);
+  m.B = 1.0 / Lb;
+  m.B_a = -m.B * m.B * Lba;
+  m.B_b = -m.B * m.B * Lbb;
+  m.p = 0.5 * sr / sb;
+  m.p_rho = 0.5 / sb;
+  m.p_beta = -m.p / sb;
+  return m;
+}
+
+// Tangent of the Chandrashekar flux with respect to both nodes' primitives
+// (the normal n is geometry, not differentiated).
+template <int DIM>
+DG_HD void chandrashekar_jvp(double inv_gm1, const NodePrims<DIM>& a, const NodePrims<DIM>& b,
+                             const PrimTangent<DIM>& da, const PrimTangent<DIM>& db,
+                             const double* n, double* df) {
+  const ChandrashekarMeans m = chandrashekar_means<DIM>(a, b);
+  double vna = 0.0, vnb = 0.0, dvna = 0.0, dvnb = 0.0;
+  for (int k = 0; k < DIM; ++k) {
+    vna += a.v[k] * n[k];
+    vnb += b.v[k] * n[k];
+    dvna += da.v[k] * n[k];
+    dvnb += db.v[k] * n[k];
+  }
+  const double drm = m.rho_a * da.rho + m.rho_b * db.rho;
+  const double dB = m.B_a * da.beta + m.B_b * db.beta;
+  const double dpm = m.p_rho * (da.rho + db.rho) + m.p_beta * (da.beta + db.beta);
+  const double vsq = a.kin + b.kin, dvsq = da.kin + db.kin;
+  const double f1 = m.rho * 0.5 * (vna + vnb);
+  const double df1 = drm * 0.5 * (vna + vnb) + m.rho * 0.5 * (dvna + dvnb);
+  const double A = 0.5 * (inv_gm1 * m.B - vsq);
+  double dfe = df1 * A + f1 * 0.5 * (inv_gm1 * dB - dvsq);
+  df[0] = df1;
+  for (int k = 0; k < DIM; ++k) {
+    const double vavg = 0.5 * (a.v[k] + b.v[k]);
+    const double dvavg = 0.5 * (da.v[k] + db.v[k]);
+    const double fk = f1 * vavg + m.p * n[k];
+    const double dfk = df1 * vavg + f1 * dvavg + dpm * n[k];
+    df[k + 1] = dfk;
+    dfe += dfk * vavg + fk * dvavg;
+  }
+  df[DIM + 1] = dfe;
+}
+
+// abar += (df/dq_a)^T g, bbar += (df/dq_b)^T g.
+template <int DIM>
+DG_HD void chandrashekar_vjp(double inv_gm1, const NodePrims<DIM>& a, const NodePrims<DIM>& b,
+                             const double* n, const double* g, PrimTangent<DIM>& abar,
+                             PrimTangent<DIM>& bbar) {
+  const ChandrashekarMeans m = chandrashekar_means<DIM>(a, b);
+  double vna = 0.0, vnb = 0.0;
+  for (int k = 0; k < DIM; ++k) {
+    vna += a.v[k] * n[k];
+    vnb += b.v[k] * n[k];
+  }
+  const double f1 = m.rho * 0.5 * (vna + vnb);
+  const double A = 0.5 * (inv_gm1 * m.B - (a.kin + b.kin));
+  const double gE = g[DIM + 1];
+  double f1b = g[0] + gE * A;
+  double pmb = 0.0;
+  for (int k = 0; k < DIM; ++k) {
+    const double vavg = 0.5 * (a.v[k] + b.v[k]);
+    const double fk = f1 * vavg + m.p * n[k];
+    const double gk = g[k + 1] + gE * vavg;  // cotangent of f_k (own and in f_E)
+    f1b += gk * vavg;
+    pmb += gk * n[k];
+    const double vavgb = gk * f1 + gE * fk;
+    abar.v[k] += 0.5 * vavgb;
+    bbar.v[k] += 0.5 * vavgb;
+  }
+  const double Bb = gE * f1 * 0.5 * inv_gm1;
+  const double vsqb = -gE * f1 * 0.5;
+  const double rmb = f1b * 0.5 * (vna + vnb);
+  const double vnb_ = f1b * m.rho * 0.5;  // cotangent of vna and of vnb
+  for (int k = 0; k < DIM; ++k) {
+    abar.v[k] += vnb_ * n[k];
+    bbar.v[k] += vnb_ * n[k];
+  }
+  abar.kin += vsqb;
+  bbar.kin += vsqb;
+  abar.rho += m.rho_a * rmb + m.p_rho * pmb;
+  bbar.rho += m.rho_b * rmb + m.p_rho * pmb;
+  abar.beta += m.B_a * Bb + m.p_beta * pmb;
+  bbar.beta += m.B_b * Bb + m.p_beta * pmb;
+}
+
+// ---------------------------------------------------------------------
+// Physical flux F(u).n and the LLF flux, in terms of u with the node
+// primitives q (rho, v, p, c) of u.
+// ---------------------------------------------------------------------
+
+// Tangent of (v, p) of a state: dv_k = (dm_k - v_k drho) / rho,
+// dp = (gamma - 1)(dE - v.dm + kin drho).
+template <int DIM>
+DG_HD void vp_jvp(double gamma, const NodePrims<DIM>& q, const double* du, double* dv, double& dp) {
+  const double ir = 1.0 / q.rho;
+  double vdm = 0.0;
+  for (int k = 0; k < DIM; ++k) {
+    dv[k] = (du[k + 1] - q.v[k] * du[0]) * ir;
+    vdm += q.v[k] * du[k + 1];
+  }
+  dp = (gamma - 1.0) * (du[DIM + 1] - vdm + q.kin * du[0]);
+}
+
+// ubar += (d(v, p)/du)^T (vbar, pbar).
+template <int DIM>
+DG_HD void vp_vjp(double gamma, const NodePrims<DIM>& q, const double* vbar, double pbar,
+                  double* ubar) {
+  const double ir = 1.0 / q.rho;
+  double rb = (gamma - 1.0) * q.kin * pbar;
+  for (int k = 0; k < DIM; ++k) {
+    ubar[k + 1] += vbar[k] * ir - (gamma - 1.0) * q.v[k] * pbar;
+    rb -= vbar[k] * q.v[k] * ir;
+  }
+  ubar[0] += rb;
+  ubar[DIM + 1] += (gamma - 1.0) * pbar;
+}
+
+// d(F(u).n) for F0 = rho vn, F_k = m_k vn + p n_k, F_E = (E + p) vn.
+template <int DIM>
+DG_HD void flux_n_jvp(double gamma, const double* u, const NodePrims<DIM>& q, const double* du,
+                      const double* n, double* dF) {
+  double dv[DIM], dp, vn = 0.0, dvn = 0.0;
+  vp_jvp<DIM>(gamma, q, du, dv, dp);
+  for (int k = 0; k < DIM; ++k) {
+    vn += q.v[k] * n[k];
+    dvn += dv[k] * n[k];
+  }
+  dF[0] = du[0] * vn + u[0] * dvn;
+  for (int k = 0; k < DIM; ++k) dF[k + 1] = du[k + 1] * vn + u[k + 1] * dvn + dp * n[k];
+  dF[DIM + 1] = (du[DIM + 1] + dp) * vn + (u[DIM + 1] + q.p) * dvn;
+}
+
+// ubar += (d(F(u).n)/du)^T g
+template <int DIM>
+DG_HD void flux_n_vjp(double gamma, const double* u, const NodePrims<DIM>& q, const double* n,
+                      const double* g, double* ubar) {
+  double vn = 0.0;
+  for (int k = 0; k < DIM; ++k) vn += q.v[k] * n[k];
+  const double gE = g[DIM + 1];
+  ubar[0] += g[0] * vn;
+  double vnb = g[0] * u[0] + gE * (u[DIM + 1] + q.p);
+  double pb = gE * vn;
+  for (int k = 0; k < DIM; ++k) {
+    ubar[k + 1] += g[k + 1] * vn;
+    vnb += g[k + 1] * u[k + 1];
+    pb += g[k + 1] * n[k];
+  }
+  ubar[DIM + 1] += gE * vn;
+  double vb[DIM];
+  for (int k = 0; k < DIM; ++k) vb[k] = vnb * n[k];
+  vp_vjp<DIM>(gamma, q, vb, pb, ubar);
+}
+
+// Gradient of the LLF wave speed max(|vn_L|, |vn_R|) + max(c_L, c_R)|n| with
+// respect to one side's state: `left` selects the side, and only the terms
+// whose max that side attains (the forward's dmax choice) contribute.
+template <int DIM>
+DG_HD void llf_lambda_grad(double gamma, const NodePrims<DIM>& ql, const NodePrims<DIM>& qr,
+                           const double* n, bool left, double* grad) {
+  double vnl = 0.0, vnr = 0.0, nn = 0.0;
+  for (int k = 0; k < DIM; ++k) {
+    vnl += ql.v[k] * n[k];
+    vnr += qr.v[k] * n[k];
+    nn += n[k] * n[k];
+  }
+  const NodePrims<DIM>& q = left ? ql : qr;
+  for (int i = 0; i < DIM + 2; ++i) grad[i] = 0.0;
+  // dmax(a, b) = a < b ? b : a picks the left argument on ties
+  const bool vn_left = !(fabs(vnl) < fabs(vnr));
+  const bool c_left = !(ql.c < qr.c);
+  double vb[DIM] = {};
+  double pb = 0.0;
+  if (vn_left == left) {
+    const double vn = left ? vnl : vnr;
+    const double s = vn > 0.0 ? 1.0 : (vn < 0.0 ? -1.0 : 0.0);
+    for (int k = 0; k < DIM; ++k) vb[k] += s * n[k];
+  }
+  if (c_left == left) {
+    // c = sqrt(gamma p / rho): dc = c/2 (dp/p - drho/rho)
+    const double w = sqrt(nn) * 0.5 * q.c;
+    pb += w / q.p;
+    grad[0] -= w / q.rho;
+  }
+  vp_vjp<DIM>(gamma, q, vb, pb, grad);
+}
+
+// Tangent of the LLF flux 0.5 (F_L + F_R).n - 0.5 lambda (u_R - u_L).
+template <int DIM>
+DG_HD void llf_jvp(double gamma, const double* ul, const NodePrims<DIM>& ql, const double* ur,
+                   const NodePrims<DIM>& qr, const double* dul, const double* dur, const double* n,
+                   double* df) {
+  constexpr int NV = DIM + 2;
+  double dfl[NV], dfr[NV], gl[NV], gr[NV];
+  flux_n_jvp<DIM>(gamma, ul, ql, dul, n, dfl);
+  flux_n_jvp<DIM>(gamma, ur, qr, dur, n, dfr);
+  llf_lambda_grad<DIM>(gamma, ql, qr, n, true, gl);
+  llf_lambda_grad<DIM>(gamma, ql, qr, n, false, gr);
+  double vnl = 0.0, vnr = 0.0, nn = 0.0;
+  for (int k = 0; k < DIM; ++k) {
+    vnl += ql.v[k] * n[k];
+    vnr += qr.v[k] * n[k];
+    nn += n[k] * n[k];
+  }
+  const double lam = dmax(fabs(vnl), fabs(vnr)) + dmax(ql.c, qr.c) * sqrt(nn);
+  double dlam = 0.0;
+  for (int i = 0; i < NV; ++i) dlam += gl[i] * dul[i] + gr[i] * dur[i];
+  for (int i = 0; i < NV; ++i)
+    df[i] = 0.5 * (dfl[i] + dfr[i]) - 0.5 * dlam * (ur[i] - ul[i]) - 0.5 * lam * (dur[i] - dul[i]);
+}
+
+// ubar_side += (df/du_side)^T g for one side of the LLF flux.
+template <int DIM>
+DG_HD void llf_vjp_side(double gamma, const double* ul, const NodePrims<DIM>& ql, const double* ur,
+                        const NodePrims<DIM>& qr, const double* n, const double* g, bool left,
+                        double* ubar) {
+  constexpr int NV = DIM + 2;
+  double vnl = 0.0, vnr = 0.0, nn = 0.0;
+  for (int k = 0; k < DIM; ++k) {
+    vnl += ql.v[k] * n[k];
+    vnr += qr.v[k] * n[k];
+    nn += n[k] * n[k];
+  }
+  const double lam = dmax(fabs(vnl), fabs(vnr)) + dmax(ql.c, qr.c) * sqrt(nn);
+  double gh[NV], lg[NV], gdu = 0.0;
+  for (int i = 0; i < NV; ++i) {
+    gh[i] = 0.5 * g[i];
+    gdu += g[i] * (ur[i] - ul[i]);
+  }
+  flux_n_vjp<DIM>(gamma, left ? ul : ur, left ? ql : qr, n, gh, ubar);
+  llf_lambda_grad<DIM>(gamma, ql, qr, n, left, lg);
+  const double sgn = left ? 0.5 : -0.5;
+  for (int i = 0; i < NV; ++i) ubar[i] += sgn * lam * g[i] - 0.5 * gdu * lg[i];
+}
+
+// ---------------------------------------------------------------------
+// Navier-Stokes: gradient variables q = (rho, v, T = p / rho) and the
+// Newtonian viscous flux (parabolic_impl.hpp grad_vars, viscous_flux_dir).
+// ---------------------------------------------------------------------
+
+template <int DIM>
+DG_HD void grad_vars_jvp(double gamma, const double* u, const double* du, double* dq) {
+  const double ir = 1.0 / u[0];
+  double m2 = 0.0, mdm = 0.0;
+  for (int k = 0; k < DIM; ++k) {
+    m2 += u[k + 1] * u[k + 1];
+    mdm += u[k + 1] * du[k + 1];
+  }
+  const double p = (gamma - 1.0) * (u[DIM + 1] - 0.5 * m2 * ir);
+  const double dp = (gamma - 1.0) * (du[DIM + 1] - mdm * ir + 0.5 * m2 * ir * ir * du[0]);
+  dq[0] = du[0];
+  for (int k = 0; k < DIM; ++k) dq[k + 1] = (du[k + 1] - u[k + 1] * ir * du[0]) * ir;
+  dq[DIM + 1] = (dp - p * ir * du[0]) * ir;
+}
+
+template <int DIM>
+DG_HD void grad_vars_vjp(double gamma, const double* u, const double* qbar, double* ubar) {
+  const double ir = 1.0 / u[0];
+  double m2 = 0.0;
+  for (int k = 0; k < DIM; ++k) m2 += u[k + 1] * u[k + 1];
+  const double p = (gamma - 1.0) * (u[DIM + 1] - 0.5 * m2 * ir);
+  const double Tb = qbar[DIM + 1];
+  // T = p / rho
+  const double pb = Tb * ir;
+  double rb = qbar[0] - Tb * p * ir * ir;
+  // p = (gamma - 1)(E - |m|^2 / (2 rho))
+  ubar[DIM + 1] += (gamma - 1.0) * pb;
+  rb += (gamma - 1.0) * 0.5 * m2 * ir * ir * pb;
+  for (int k = 0; k < DIM; ++k) {
+    ubar[k + 1] += -(gamma - 1.0) * u[k + 1] * ir * pb + qbar[k + 1] * ir;
+    rb -= qbar[k + 1] * u[k + 1] * ir * ir;
+  }
+  ubar[0] += rb;
+}
+
+// Symmetric viscous stress tau (without mu) from the velocity gradients
+// g[c][1 + k] = d v_k / d x_c, as viscous_flux_dir forms it.
+template <int DIM>
+DG_HD void stress(const double g[DIM][DIM + 2], double tau[DIM][DIM]) {
+  if constexpr (DIM == 2) {
+    tau[0][0] = (4.0 * g[0][1] - 2.0 * g[1][2]) / 3.0;
+    tau[1][1] = (4.0 * g[1][2] - 2.0 * g[0][1]) / 3.0;
+    tau[0][1] = tau[1][0] = g[1][1] + g[0][2];
+  } else {
+    tau[0][0] = (4.0 * g[0][1] - 2.0 * (g[1][2] + g[2][3])) / 3.0;
+    tau[1][1] = (4.0 * g[1][2] - 2.0 * (g[0][1] + g[2][3])) / 3.0;
+    tau[2][2] = (4.0 * g[2][3] - 2.0 * (g[0][1] + g[1][2])) / 3.0;
+    tau[0][1] = tau[1][0] = g[1][1] + g[0][2];
+    tau[0][2] = tau[2][0] = g[2][1] + g[0][3];
+    tau[1][2] = tau[2][1] = g[2][2] + g[1][3];
+  }
+}
+
+// gbar += (dtau/dg)^T taubar (taubar as a full, not necessarily symmetric,
+// matrix of cotangents of the entries tau[c][k]).
+template <int DIM>
+DG_HD void stress_vjp(const double taubar[DIM][DIM], double gbar[DIM][DIM + 2]) {
+  if constexpr (DIM == 2) {
+    const double t00 = taubar[0][0], t11 = taubar[1][1], t01 = taubar[0][1] + taubar[1][0];
+    gbar[0][1] += (4.0 * t00 - 2.0 * t11) / 3.0;
+    gbar[1][2] += (4.0 * t11 - 2.0 * t00) / 3.0;
+    gbar[1][1] += t01;
+    gbar[0][2] += t01;
+  } else {
+    const double t00 = taubar[0][0], t11 = taubar[1][1], t22 = taubar[2][2];
+    const double t01 = taubar[0][1] + taubar[1][0];
+    const double t02 = taubar[0][2] + taubar[2][0];
+    const double t12 = taubar[1][2] + taubar[2][1];
+    gbar[0][1] += (4.0 * t00 - 2.0 * (t11 + t22)) / 3.0;
+    gbar[1][2] += (4.0 * t11 - 2.0 * (t00 + t22)) / 3.0;
+    gbar[2][3] += (4.0 * t22 - 2.0 * (t00 + t11)) / 3.0;
+    gbar[1][1] += t01;
+    gbar[0][2] += t01;
+    gbar[2][1] += t02;
+    gbar[0][3] += t02;
+    gbar[2][2] += t12;
+    gbar[1][3] += t12;
+  }
+}
+
+// Tangent of the viscous flux in every direction c: f[c] = (0, mu tau[c][.],
+// mu (v . tau[c][.] + kappa dT/dx_c)).
+template <int DIM>
+DG_HD void viscous_flux_jvp(double mu, double kappa, const double* q, const double g[DIM][DIM + 2],
+                            const double* dq, const double dg[DIM][DIM + 2],
+                            double df[DIM][DIM + 2]) {
+  double tau[DIM][DIM], dtau[DIM][DIM];
+  stress<DIM>(g, tau);
+  stress<DIM>(dg, dtau);
+  for (int c = 0; c < DIM; ++c) {
+    df[c][0] = 0.0;
+    double e = kappa * dg[c][DIM + 1];
+    for (int k = 0; k < DIM; ++k) {
+      df[c][k + 1] = mu * dtau[c][k];
+      e += dq[k + 1] * tau[c][k] + q[k + 1] * dtau[c][k];
+    }
+    df[c][DIM + 1] = mu * e;
+  }
+}
+
+// qbar += (df/dq)^T fbar, gbar += (df/dg)^T fbar over all directions c.
+template <int DIM>
+DG_HD void viscous_flux_vjp(double mu, double kappa, const double* q, const double g[DIM][DIM + 2],
+                            const double fbar[DIM][DIM + 2], double* qbar,
+                            double gbar[DIM][DIM + 2]) {
+  double tau[DIM][DIM], taubar[DIM][DIM];
+  stress<DIM>(g, tau);
+  for (int c = 0; c < DIM; ++c) {
+    const double fe = mu * fbar[c][DIM + 1];
+    gbar[c][DIM + 1] += kappa * fe;
+    for (int k = 0; k < DIM; ++k) {
+      taubar[c][k] = mu * fbar[c][k + 1] + fe * q[k + 1];
+      qbar[k + 1] += fe * tau[c][k];
+    }
+  }
+  stress_vjp<DIM>(taubar, gbar);
+}
+
+}  // namespace dgsem

@@ -183,6 +183,53 @@ class StructuredSolver:
     def ssprk3_steps(self, u, dt: float, nsteps: int, tmp1, tmp2) -> None:
         self._check(self._L.dgsem_ssprk3_steps(self._h, _ptr(u), float(dt), int(nsteps), _ptr(tmp1), _ptr(tmp2)))
 
+    # -- tangent / adjoint (no reference counterpart) ------------------------
+    def rhs_jvp(self, u, du, out=None):
+        """out = J(u) du, J = dL/du."""
+        out = self.empty() if out is None else out
+        self._check(self._L.dgsem_rhs_jvp(self._h, _ptr(u), _ptr(du), _ptr(out)))
+        return out
+
+    def rhs_vjp(self, u, w, out=None):
+        """out = J(u)^T w."""
+        out = self.empty() if out is None else out
+        self._check(self._L.dgsem_rhs_vjp(self._h, _ptr(u), _ptr(w), _ptr(out)))
+        return out
+
+    def ssprk3_stages(self, u0, dt: float, u1=None, u2=None):
+        """Stage states u1, u2 of one SSPRK3 step from u0."""
+        u1 = self.empty() if u1 is None else u1
+        u2 = self.empty() if u2 is None else u2
+        self.ssprk3_stage(1, dt, None, u0, u1)
+        self.ssprk3_stage(2, dt, u0, u1, u2)
+        return u1, u2
+
+    def step_jvp(self, dt: float, u0, u1, u2, du0, out=None, tmp1=None, tmp2=None):
+        out = self.empty() if out is None else out
+        tmp1 = self.empty() if tmp1 is None else tmp1
+        tmp2 = self.empty() if tmp2 is None else tmp2
+        self._check(self._L.dgsem_ssprk3_step_jvp(self._h, float(dt), _ptr(u0), _ptr(u1), _ptr(u2),
+                                                   _ptr(du0), _ptr(out), _ptr(tmp1), _ptr(tmp2)))
+        return out
+
+    def step_vjp(self, dt: float, u0, u1, u2, ubar, out=None, tmp1=None, tmp2=None):
+        out = self.empty() if out is None else out
+        tmp1 = self.empty() if tmp1 is None else tmp1
+        tmp2 = self.empty() if tmp2 is None else tmp2
+        self._check(self._L.dgsem_ssprk3_step_vjp(self._h, float(dt), _ptr(u0), _ptr(u1), _ptr(u2),
+                                                   _ptr(ubar), _ptr(out), _ptr(tmp1), _ptr(tmp2)))
+        return out
+
+    def kinetic_energy(self, u, result=None):
+        """Kinetic-energy quadrature of u into a one-element device tensor."""
+        result = torch.empty(1, dtype=torch.float64, device="cuda") if result is None else result
+        self._check(self._L.dgsem_kinetic_energy(self._h, _ptr(u), _ptr(result)))
+        return result
+
+    def kinetic_energy_grad(self, u, scale: float, grad) -> None:
+        """grad += scale * d(kinetic energy)/du."""
+        self._check(self._L.dgsem_kinetic_energy_grad(self._h, _ptr(u), float(scale), _ptr(grad)))
+
     def synchronize(self) -> None:
         self._check(self._L.dgsem_synchronize(self._h))
 

@@ -164,6 +164,37 @@ int dgsem_ssprk3_steps(dgsem_handle* h, double* u, double dt, int nsteps, double
 int dgsem_synchronize(dgsem_handle* h);
 
 /* ---------------------------------------------------------------------
+ * Tangent and discrete adjoint.  The reference has no derivatives; these
+ * differentiate the right-hand side this library computes (split form,
+ * LLF surface flux, Euler or Navier-Stokes with BR1; the shock-capturing
+ * blend and the positivity floor are not differentiated) exactly, up to
+ * rounding, with J = dL/du at the state u:
+ * --------------------------------------------------------------------- */
+/* out = J(u) du */
+int dgsem_rhs_jvp(dgsem_handle* h, const double* u, const double* du, double* out);
+/* out = J(u)^T w */
+int dgsem_rhs_vjp(dgsem_handle* h, const double* u, const double* w, double* out);
+/* Tangent of one SSPRK3 step (the map u0 -> u3 of dgsem_ssprk3_stage 1..3)
+ * given its stage states u0, u1, u2 (from dgsem_ssprk3_stage): du_out =
+ * (du3/du0) du0.  tmp1, tmp2: scratch arrays of the state's size. */
+int dgsem_ssprk3_step_jvp(dgsem_handle* h, double dt, const double* u0, const double* u1,
+                          const double* u2, const double* du0, double* du_out, double* tmp1,
+                          double* tmp2);
+/* Adjoint of one SSPRK3 step: ubar_out = (du3/du0)^T ubar.  ubar_out may
+ * alias ubar. */
+int dgsem_ssprk3_step_vjp(dgsem_handle* h, double dt, const double* u0, const double* u1,
+                          const double* u2, const double* ubar, double* ubar_out, double* tmp1,
+                          double* tmp2);
+/* Kinetic energy sum over nodes of w_i w_j (w_k) / |1/J| |rho v|^2 / (2 rho),
+ * the quadrature of VolumeAverageEuler with the volume weights of
+ * VolumeWeightedAnalyzerFunctor (volume_average_euler.hpp:30-53,
+ * volume_weighted_analyzer_functor.hpp:108-134), reduced in a fixed order
+ * into the device double *result. */
+int dgsem_kinetic_energy(dgsem_handle* h, const double* u, double* result);
+/* grad += scale * d(kinetic energy)/du */
+int dgsem_kinetic_energy_grad(dgsem_handle* h, const double* u, double scale, double* grad);
+
+/* ---------------------------------------------------------------------
  * Batched ensembles.  The reference has no ensemble API: its drivers run
  * one StructuredSolver per process, one case after another.  A batch
  * advances many independent 2D Euler cases (different element grids and
