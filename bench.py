@@ -37,14 +37,8 @@ C_PAIR_3D, C_NODE_3D = 107, 15
 def tgv_state(coords: np.ndarray, gamma: float = 1.4) -> np.ndarray:
     """Taylor-Green vortex (Mach 0.1) in conservative variables, reference
     layout u[e][dof][var]."""
-    x, y, z = coords[..., 0], coords[..., 1], coords[..., 2]
-    rho = np.ones_like(x)
-    v1 = np.sin(x) * np.cos(y) * np.cos(z)
-    v2 = -np.cos(x) * np.sin(y) * np.cos(z)
-    p = 100.0 / gamma + (1.0 / 16.0) * (np.cos(2 * x) * np.cos(2 * z) + 2 * np.cos(2 * y)
-                                       + 2 * np.cos(2 * x) + np.cos(2 * y) * np.cos(2 * z))
-    E = p / (gamma - 1.0) + 0.5 * rho * (v1 * v1 + v2 * v2)
-    return np.stack([rho, rho * v1, rho * v2, np.zeros_like(x), E], axis=-1)
+    from dgsem_amd import tgv
+    return tgv.tgv_state(coords, gamma)
 
 
 def volume_flops(E: int, N: int) -> float:
@@ -90,6 +84,79 @@ def cpu_baseline(N: int, cells: int, steps: int) -> dict:
             "ms_per_step": t_step * 1e3}
 
 
+def batch_measure(ncases: int = 256, steps: int = 100, seed: int = 2024) -> dict:
+    """Ensemble of 2D Kelvin-Helmholtz cases (grids from {8,16,32,64}^2, N from
+    {2,3,4,5}): `steps` SSPRK3 steps of the autotuned batch (graph replay of
+    one persistent launch per stage) against the same steps run case after
+    case with the single-case handles; per-case results compared bitwise."""
+    import torch
+
+    from dgsem_amd import ensemble
+
+    cases = ensemble.khi_ensemble(seed=seed, ncases=ncases)
+    batch, u0, dts = ensemble.khi_batch(cases)
+    nodes = sum(batch.case(i).n_elements * batch.case(i).ndofs for i in range(ncases))
+    tuning = batch.autotune(u0, batch.empty(), reps=3)
+    ub, t1, t2 = u0.clone(), batch.empty(), batch.empty()
+    batch.ssprk3_steps(ub, 1, t1, t2)  # graph capture and warm-up
+    ub.copy_(u0)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record()
+    batch.ssprk3_steps(ub, steps, t1, t2)
+    ev1.record()
+    torch.cuda.synchronize()
+    t_batch = ev0.elapsed_time(ev1)
+    singles = []
+    for i in range(ncases):
+        s = batch.case(i)
+        singles.append((s, batch.view(u0, i).clone(), s.empty(), s.empty()))
+    s, u, a, b = singles[0]
+    s.ssprk3_steps(u.clone(), dts[0], 1, a, b)  # warm-up
+    torch.cuda.synchronize()
+    ev0.record()
+    for i, (s, u, a, b) in enumerate(singles):
+        s.ssprk3_steps(u, dts[i], steps, a, b)
+    ev1.record()
+    torch.cuda.synchronize()
+    t_seq = ev0.elapsed_time(ev1)
+    identical = all(torch.equal(batch.view(ub, i), singles[i][1]) for i in range(ncases))
+    return {"workload": f"khi2d_ensemble_{ncases}_cases_{steps}_steps_seed{seed}",
+            "nodes": nodes, "batched_ms": t_batch, "sequential_ms": t_seq,
+            "speedup": t_seq / t_batch, "bit_identical": identical,
+            "dof_updates_per_s": nodes * 3 * steps / (t_batch * 1e-3), "tuning": tuning}
+
+
+def adjoint_measure(cells: int = 16, N: int = 3, steps: int = 200, every: int = 20) -> dict:
+    """3D Navier-Stokes TGV (Re 1600, Pr 0.72): time of the forward trajectory
+    with the kinetic-energy objective against the checkpointed reverse sweep
+    (recompute between checkpoints + adjoint steps)."""
+    import torch
+
+    import dgsem_amd as dg
+    from dgsem_amd import adjoint, tgv
+
+    cfg = dg.SolverConfig(dim=3, polydeg=N, cells=(cells,) * 3, domain_left=(0, 0, 0),
+                          domain_right=(2 * math.pi,) * 3, equation=dg.NAVIER_STOKES,
+                          mu=6.25e-4, prandtl=0.72)
+    s = dg.StructuredSolver(cfg)
+    u0 = s.from_reference_layout(tgv.tgv_state(s.node_coordinates()))
+    dt = tgv.tgv_dt(cells, N)
+    adjoint.kinetic_energy_objective(s, u0, dt, 4, 2)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J, ckpt = adjoint.kinetic_energy_trajectory(s, u0, dt, steps, every)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    adjoint.kinetic_energy_gradient(s, ckpt, dt, steps, every)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    return {"workload": f"tgv3d_navier_stokes_{cells}^3_N{N}_{steps}_steps_checkpoint_{every}",
+            "forward_ms_per_step": (t1 - t0) / steps * 1e3,
+            "adjoint_ms_per_step": (t2 - t1) / steps * 1e3,
+            "ratio": (t2 - t1) / (t1 - t0), "objective": J}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +168,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-cells", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched-ensemble and adjoint measurements")
     ap.add_argument("--profile-copy-mib", type=int, default=0,
                     help="after the run, launch an elementwise copy kernel of this size "
                          "(calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -179,6 +248,10 @@ def main() -> int:
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "flop_per_launch": flops, "kernel_ms": t_kernel_ms},
     }
+    if not args.no_extras:
+        out["batch"] = batch_measure()
+        out["adjoint"] = adjoint_measure()
+        out["adjoint_forward_step_ratio"] = out["adjoint"]["ratio"]
     if args.profile_copy_mib > 0:
         a = torch.ones(args.profile_copy_mib * (1 << 20) // 8, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
