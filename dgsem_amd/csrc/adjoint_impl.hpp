@@ -286,12 +286,15 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) jvp_rhs_kernel(c
     auto ja = [&](int64_t el, int dof, double* out) {
       for (int c = 0; c < DIM; ++c) out[c] = metric<NODAL, DIM, NN>(p, el, dir, c, dof);
     };
+#pragma unroll
     for (int ia = 0; ia < n - 1; ++ia) {
       const NodePrims<DIM> qa = prims(dofs[ia]);
       const PrimTangent<DIM> ta = tangent(dofs[ia]);
       double jaa[DIM];
       ja(e, dofs[ia], jaa);
+#pragma unroll
       for (int ib = ia + 1; ib < n; ++ib) {
+        __builtin_amdgcn_sched_barrier(0);
         const NodePrims<DIM> qb = prims(dofs[ib]);
         const PrimTangent<DIM> tb = tangent(dofs[ib]);
         double jab[DIM], nrm[DIM], df[NV];
@@ -306,6 +309,8 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) jvp_rhs_kernel(c
       }
     }
     // LLF faces, both states and their tangents from memory
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll 1
     for (int side = 0; side < 2; ++side) {
       const bool lower = side == 0;
       const int64_t nb = p.nbr[e * 2 * DIM + 2 * dir + side];
@@ -324,8 +329,10 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) jvp_rhs_kernel(c
       llf_jvp<DIM>(p.gamma, ul, node_prims<DIM>(p.gamma, ul), ur, node_prims<DIM>(p.gamma, ur), dul,
                    dur, nrm, df);
       if (lower) {
+#pragma unroll
         for (int v = 0; v < NV; ++v) acc[0][v] -= df[v] * p.lhat0;
       } else {
+#pragma unroll
         for (int v = 0; v < NV; ++v) acc[n - 1][v] += df[v] * p.lhatN;
       }
     }
@@ -574,11 +581,16 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(c
     auto ja = [&](int64_t el, int dof, double* out) {
       for (int c = 0; c < DIM; ++c) out[c] = metric<NODAL, DIM, NN>(p, el, dir, c, dof);
     };
+#pragma unroll
     for (int ia = 0; ia < n - 1; ++ia) {
       const NodePrims<DIM> qa = prims(dofs[ia]);
       double jaa[DIM];
       ja(e, dofs[ia], jaa);
+#pragma unroll
       for (int ib = ia + 1; ib < n; ++ib) {
+        // one pair at a time: interleaving pairs would keep several
+        // adjoint flux evaluations live at once
+        __builtin_amdgcn_sched_barrier(0);
         const NodePrims<DIM> qbn = prims(dofs[ib]);
         double jab[DIM], nrm[DIM], g[NV];
         ja(e, dofs[ib], jab);
@@ -589,6 +601,8 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(c
       }
     }
     // LLF faces: this element's share of each face flux's adjoint
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll 1
     for (int side = 0; side < 2; ++side) {
       const bool lower = side == 0;
       const int64_t nb = p.nbr[e * 2 * DIM + 2 * dir + side];
@@ -605,8 +619,15 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(c
       // the face flux enters the upper element's first node with -l_hat0
       // and the lower element's last node with +l_hatN
       for (int v = 0; v < NV; ++v) g[v] = p.lhatN * rl[v] - p.lhat0 * rrv[v];
+      double ubf[NV];
+      for (int v = 0; v < NV; ++v) ubf[v] = 0.0;
       llf_vjp_side<DIM>(p.gamma, ul, node_prims<DIM>(p.gamma, ul), ur, node_prims<DIM>(p.gamma, ur),
-                        nrm, g, !lower, lower ? ub_lo : ub_hi);
+                        nrm, g, !lower, ubf);
+      if (lower) {
+        for (int v = 0; v < NV; ++v) ub_lo[v] = ubf[v];
+      } else {
+        for (int v = 0; v < NV; ++v) ub_hi[v] = ubf[v];
+      }
     }
   }
   for (int d = 0; d < DIM; ++d) {

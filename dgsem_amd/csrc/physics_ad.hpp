@@ -62,27 +62,29 @@ DG_HD void prims_vjp(double gamma, const NodePrims<DIM>& q, const PrimTangent<DI
   ubar[0] += rb;
 }
 
-// Logarithmic mean and its partial derivatives.  `series` selects the
-// branch the forward evaluation took.
+// Logarithmic mean and its partial derivatives on both branches, the one
+// the forward evaluation took selected by `series`.  Both are computed and
+// one is selected without a branch: divergent branches in each of the
+// unrolled pair evaluations of a line would keep the line's state live across
+// all of them and multiply the register count.  On the series branch the log
+// form may be inf / nan (x == y); it is discarded by the select.
 DG_HD void ln_mean_partials(double x, double y, bool series, double& L, double& Lx, double& Ly) {
-  if (series) {
-    const double s = x + y, d = x - y;
-    const double is = 1.0 / s;
-    const double f = d * d * is * is;
-    const double P = 2.0 + f * (2.0 / 3.0 + f * (2.0 / 5.0 + f * (2.0 / 7.0)));
-    const double dP = 2.0 / 3.0 + f * (4.0 / 5.0 + f * (6.0 / 7.0));
-    const double fx = 2.0 * d * is * is - 2.0 * d * d * is * is * is;
-    const double fy = -2.0 * d * is * is - 2.0 * d * d * is * is * is;
-    L = s / P;
-    const double c = s * dP / (P * P);
-    Lx = 1.0 / P - c * fx;
-    Ly = 1.0 / P - c * fy;
-  } else {
-    const double D = log(y / x);
-    L = (y - x) / D;
-    Lx = (L / x - 1.0) / D;
-    Ly = (1.0 - L / y) / D;
-  }
+  const double s = x + y, d = x - y;
+  const double is = 1.0 / s;
+  const double f = d * d * is * is;
+  const double P = 2.0 + f * (2.0 / 3.0 + f * (2.0 / 5.0 + f * (2.0 / 7.0)));
+  const double dP = 2.0 / 3.0 + f * (4.0 / 5.0 + f * (6.0 / 7.0));
+  const double fx = 2.0 * d * is * is - 2.0 * d * d * is * is * is;
+  const double fy = -2.0 * d * is * is - 2.0 * d * d * is * is * is;
+  const double Ls = s / P;
+  const double c = s * dP / (P * P);
+  const double Lxs = 1.0 / P - c * fx, Lys = 1.0 / P - c * fy;
+  const double D = log(y / x);
+  const double Ll = (y - x) / D;
+  const double Lxl = (Ll / x - 1.0) / D, Lyl = (1.0 - Ll / y) / D;
+  L = series ? Ls : Ll;
+  Lx = series ? Lxs : Lxl;
+  Ly = series ? Lys : Lyl;
 }
 
 // Means of the Chandrashekar flux and their partial derivatives with
