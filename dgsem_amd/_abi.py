@@ -26,7 +26,7 @@ EXPORTED = (
     "dgsem_batch_case", "dgsem_batch_set_dt", "dgsem_batch_rhs", "dgsem_batch_ssprk3_steps",
     "dgsem_batch_get_tuning", "dgsem_batch_set_tuning", "dgsem_batch_autotune",
     "dgsem_rhs_jvp", "dgsem_rhs_vjp", "dgsem_ssprk3_step_jvp", "dgsem_ssprk3_step_vjp",
-    "dgsem_kinetic_energy", "dgsem_kinetic_energy_grad",
+    "dgsem_kinetic_energy", "dgsem_kinetic_energy_grad", "dgsem_analyze",
 )
 
 
@@ -97,6 +97,7 @@ def load(path: str | None = None):
     L.dgsem_ssprk3_step_vjp.argtypes = [vp, C.c_double, vp, vp, vp, vp, vp, vp, vp]
     L.dgsem_kinetic_energy.argtypes = [vp, vp, vp]
     L.dgsem_kinetic_energy_grad.argtypes = [vp, vp, C.c_double, vp]
+    L.dgsem_analyze.argtypes = [vp, vp, vp]
     for name in EXPORTED:
         if name not in ("dgsem_config_default", "dgsem_last_error", "dgsem_batch_last_error"):
             getattr(L, name).restype = C.c_int

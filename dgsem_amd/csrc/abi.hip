@@ -64,7 +64,9 @@ void release(dgsem_handle* h) {
   hipFree(h->d_ad_qbar);
   hipFree(h->d_wq);
   hipFree(h->d_ke_partial);
+  hipFree(h->d_an_partial);
   h->d_ad_dgrad = h->d_ad_dvisc = h->d_ad_qbar = h->d_wq = h->d_ke_partial = nullptr;
+  h->d_an_partial = nullptr;
   h->d_Ja = h->d_invJ = h->d_sn = h->d_Vinv = h->d_alpha = h->d_grad = h->d_visc = nullptr;
   h->d_nbr = nullptr;
 }

@@ -189,7 +189,7 @@ int ahip(dgsem_handle* h, hipError_t e, const char* where) {
   return afail(h, DGSEM_E_HIP, std::string(where) + ": " + hipGetErrorString(e));
 }
 
-int alloc_zero(dgsem_handle* h, double** p, size_t count) {
+int alloc_once(dgsem_handle* h, double** p, size_t count) {
   if (*p) return DGSEM_OK;
   if (hipMalloc((void**)p, count * sizeof(double)) != hipSuccess)
     return afail(h, DGSEM_E_ALLOC, "hipMalloc failed");
@@ -208,8 +208,8 @@ int ad_prepare(dgsem_handle* h) {
   if (c.equation == DGSEM_NAVIER_STOKES) {
     int rc;
     const size_t cell = (size_t)h->E * h->nvars * h->nn;
-    if ((rc = alloc_zero(h, &h->d_ad_dgrad, cell * c.dim)) ||
-        (rc = alloc_zero(h, &h->d_ad_dvisc, cell)) || (rc = alloc_zero(h, &h->d_ad_qbar, cell)))
+    if ((rc = alloc_once(h, &h->d_ad_dgrad, cell * c.dim)) ||
+        (rc = alloc_once(h, &h->d_ad_dvisc, cell)) || (rc = alloc_once(h, &h->d_ad_qbar, cell)))
       return rc;
   }
   return DGSEM_OK;
@@ -253,6 +253,12 @@ int apply(dgsem_handle* h, bool adjoint, const double* u, const double* v, const
   return e == hipSuccess ? DGSEM_OK : ahip(h, e, adjoint ? "adjoint kernel launch" : "tangent kernel launch");
 }
 
+bool distinct(const void* a, const void* b) { return a && b && a != b; }
+
+}  // namespace
+
+namespace dgsem {
+
 int prepare_weights(dgsem_handle* h) {
   if (h->d_wq) return DGSEM_OK;
   const auto& w = h->basis.weights;
@@ -273,9 +279,7 @@ int prepare_weights(dgsem_handle* h) {
   return DGSEM_OK;
 }
 
-bool distinct(const void* a, const void* b) { return a && b && a != b; }
-
-}  // namespace
+}  // namespace dgsem
 
 extern "C" {
 
@@ -330,7 +334,7 @@ int dgsem_ssprk3_step_vjp(dgsem_handle* h, double dt, const double* u0, const do
 
 int dgsem_kinetic_energy(dgsem_handle* h, const double* u, double* result) {
   if (!h || !h->d_Ja || !u || !result) return DGSEM_E_INVALID;
-  int rc = prepare_weights(h);
+  int rc = dgsem::prepare_weights(h);
   if (rc) return rc;
   dgsem::KParams p = h->base;
   p.u = u;
@@ -342,7 +346,7 @@ int dgsem_kinetic_energy(dgsem_handle* h, const double* u, double* result) {
 
 int dgsem_kinetic_energy_grad(dgsem_handle* h, const double* u, double scale, double* grad) {
   if (!h || !h->d_Ja || !u || !grad || u == grad) return DGSEM_E_INVALID;
-  int rc = prepare_weights(h);
+  int rc = dgsem::prepare_weights(h);
   if (rc) return rc;
   dgsem::KParams p = h->base;
   p.u = u;

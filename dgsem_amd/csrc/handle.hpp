@@ -37,6 +37,7 @@ struct dgsem_handle {
   double* d_ad_qbar = nullptr;
   double* d_wq = nullptr;
   double* d_ke_partial = nullptr;
+  double* d_an_partial = nullptr;  // per-block partials of dgsem_analyze
   std::string err;
 };
 
@@ -51,5 +52,9 @@ int stage_params(dgsem_handle* h, int stage, double dt, const double* u_base, co
 
 // Enqueues the kernels for one evaluation with parameters p.
 int launch_params(dgsem_handle* h, const KParams& p);
+
+// Uploads the tensor-product quadrature weight of each node (h->d_wq) and
+// the reduction scratch on first use.
+int prepare_weights(dgsem_handle* h);
 
 }  // namespace dgsem

@@ -196,6 +196,20 @@ DG_HD NodePrims<DIM> node_prims(double gamma, const double* u) {
   return q;
 }
 
+// 1 / x.  DGSEM_FAST_RCP (experiment): the hardware reciprocal refined by two
+// Newton steps instead of the IEEE division sequence.
+DG_HD double recip(double x) {
+#if defined(DGSEM_FAST_RCP) && defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+#else
+  return 1.0 / x;
+#endif
+}
+
 // Chandrashekar two-point flux from hoisted node quantities,
 // numerical_flux.hpp:436-500 (2D) and :520-594 (3D).  inv_gm1 = 1/(gamma-1).
 //
@@ -220,13 +234,13 @@ DG_HD void flux_chandrashekar(double inv_gm1, const NodePrims<DIM>& a, const Nod
   const double dr = a.rho * (a.rho + 2.0 * b.rho) + b.rho * b.rho;
   const double nb = a.beta * (a.beta - 2.0 * b.beta) + b.beta * b.beta;
   const double db = a.beta * (a.beta + 2.0 * b.beta) + b.beta * b.beta;
-  const double r1 = 1.0 / (dr * db);
+  const double r1 = recip(dr * db);
   const double f2r = nr * (db * r1), f2b = nb * (dr * r1);
   const double polyr = 2.0 + f2r * (2.0 / 3.0 + f2r * (2.0 / 5.0 + f2r * (2.0 / 7.0)));
   const double polyb = 2.0 + f2b * (2.0 / 3.0 + f2b * (2.0 / 5.0 + f2b * (2.0 / 7.0)));
   double rho_mean, inv_beta_mean, p_mean;
   if (f2r < 1.0e-4 && f2b < 1.0e-4) {
-    const double r2 = 1.0 / (polyr * sb);
+    const double r2 = recip(polyr * sb);
     rho_mean = sr * (sb * r2);
     inv_beta_mean = polyb * (polyr * r2);
     p_mean = 0.5 * sr * (polyr * r2);

@@ -230,6 +230,18 @@ class StructuredSolver:
         """grad += scale * d(kinetic energy)/du."""
         self._check(self._L.dgsem_kinetic_energy_grad(self._h, _ptr(u), float(scale), _ptr(grad)))
 
+    def analyze(self, u) -> dict:
+        """One-pass analysis of u (synchronises): volume, kinetic-energy and
+        entropy integrals, their volume means, the number of non-finite
+        nodes and the largest signal speed max_d |v_d| + c."""
+        out = torch.empty(5, dtype=torch.float64, device="cuda")
+        self._check(self._L.dgsem_analyze(self._h, _ptr(u), _ptr(out)))
+        vol, ke, ent, bad, vmax = out.cpu().tolist()
+        return {"volume": vol, "kinetic_energy": ke, "entropy": ent,
+                "mean_kinetic_energy": ke / vol if vol else float("nan"),
+                "mean_entropy": ent / vol if vol else float("nan"),
+                "nonfinite_nodes": int(bad), "max_speed": vmax}
+
     def synchronize(self) -> None:
         self._check(self._L.dgsem_synchronize(self._h))
 

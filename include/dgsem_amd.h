@@ -194,6 +194,17 @@ int dgsem_kinetic_energy(dgsem_handle* h, const double* u, double* result);
 /* grad += scale * d(kinetic energy)/du */
 int dgsem_kinetic_energy_grad(dgsem_handle* h, const double* u, double scale, double* grad);
 
+/* Analysis of a state in one pass (the reference's analyzers:
+ * VolumeAverageEuler, volume_average_euler.hpp:30-53, with the volume weights
+ * w_i w_j (w_k) / |1/J| of VolumeWeightedAnalyzerFunctor,
+ * volume_weighted_analyzer_functor.hpp:108-134; DivergenceChecker,
+ * divergence_checker.hpp:6-27).  Writes 5 doubles to the device array out:
+ *   [0] total volume, [1] kinetic-energy integral, [2] entropy integral
+ *   (-rho (ln p - gamma ln rho) / (gamma - 1)), [3] number of non-finite
+ *   nodes (skipped in the sums), [4] max over nodes of max_d |v_d| + c.
+ * Reduced in a fixed order. */
+int dgsem_analyze(dgsem_handle* h, const double* u, double* out);
+
 /* ---------------------------------------------------------------------
  * Batched ensembles.  The reference has no ensemble API: its drivers run
  * one StructuredSolver per process, one case after another.  A batch

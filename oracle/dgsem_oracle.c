@@ -1651,3 +1651,49 @@ double orc_kinetic_energy(const orc_case* c, const double* u) {
     }
   return total;
 }
+
+
+/* VolumeAverageEuler (volume_average_euler.hpp:30-53) over the volume
+ * weights of VolumeWeightedAnalyzerFunctor (volume_weighted_analyzer_
+ * functor.hpp:108-134), DivergenceChecker's non-finite test
+ * (divergence_checker.hpp:6-27) and the largest max_d |v_d| + c, in element
+ * order: out = [volume, kinetic energy, entropy, non-finite nodes, max speed].
+ * Non-finite nodes are left out of the sums. */
+void orc_analyze(const orc_case* c, const double* u, double* out) {
+  const int n = c->n, d = c->dim;
+  double vol = 0.0, ke = 0.0, ent = 0.0, bad = 0.0, vmax = 0.0;
+  for (int64_t e = 0; e < c->E; ++e)
+    for (int dof = 0; dof < c->nn; ++dof) {
+      int finite = 1;
+      for (int v = 0; v < c->nvars; ++v)
+        if (!isfinite(U(u, e, dof, v))) finite = 0;
+      if (!finite) {
+        bad += 1.0;
+        continue;
+      }
+      const int i = dof % n, j = (dof / n) % n, k = dof / (n * n);
+      double w = c->weights[i] * c->weights[j];
+      if (d == 3) w = w * c->weights[k];
+      const double vw = w / fabs(INVJ(e, dof));
+      const double rho = U(u, e, dof, 0);
+      double m2 = 0.0, v2 = 0.0, vabs = 0.0;
+      for (int k2 = 0; k2 < d; ++k2) {
+        const double m = U(u, e, dof, k2 + 1), vk = m / rho;
+        m2 += m * m;
+        v2 += vk * vk;
+        if (fabs(vk) > vabs) vabs = fabs(vk);
+      }
+      const double p = (c->gamma - 1.0) * (U(u, e, dof, d + 1) - 0.5 * rho * v2);
+      const double s = log(p) - c->gamma * log(rho);
+      vol += vw;
+      ke += vw * (0.5 * m2 / rho);
+      ent += vw * (-rho * s / (c->gamma - 1.0));
+      const double sp = vabs + sqrt(c->gamma * p / rho);
+      if (sp > vmax) vmax = sp;
+    }
+  out[0] = vol;
+  out[1] = ke;
+  out[2] = ent;
+  out[3] = bad;
+  out[4] = vmax;
+}

@@ -105,6 +105,8 @@ void orc_physical_flux(int dim, double gamma, const double* u,
 /* Volume-weighted kinetic energy sum (volume_average_euler.hpp:30-53 with the
  * weights of volume_weighted_analyzer_functor.hpp:108-134). */
 double orc_kinetic_energy(const orc_case* c, const double* u);
+/* [volume, kinetic energy, entropy, non-finite nodes, max |v_d| + c] */
+void orc_analyze(const orc_case* c, const double* u, double* out);
 
 #ifdef __cplusplus
 }

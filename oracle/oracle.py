@@ -77,6 +77,7 @@ def lib():
     L.orc_physical_flux.argtypes = [C.c_int, C.c_double, _dp, _dp, _dp]
     L.orc_kinetic_energy.restype = C.c_double
     L.orc_kinetic_energy.argtypes = [vp, _dp]
+    L.orc_analyze.argtypes = [vp, _dp, _dp]
     _lib = L
     return L
 
@@ -249,3 +250,8 @@ class Case:
 
     def kinetic_energy(self, u):
         return lib().orc_kinetic_energy(self._h, np.ascontiguousarray(u, dtype=np.float64))
+
+    def analyze(self, u):
+        out = np.zeros(5)
+        lib().orc_analyze(self._h, np.ascontiguousarray(u, dtype=np.float64), out)
+        return out
