@@ -32,6 +32,21 @@ enum FluxKind : int { kFluxLLF = 0, kFluxHLLC = 1 };
 DG_HD double dmax(double a, double b) { return (a < b) ? b : a; }
 DG_HD double dmin(double a, double b) { return (b < a) ? b : a; }
 
+// 1 / x: on the GPU the hardware reciprocal refined by two Newton steps,
+// which is within an ulp of the IEEE quotient for half the instructions of
+// the division sequence (the parity tests bound the effect on the RHS).
+DG_HD double recip(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+#else
+  return 1.0 / x;
+#endif
+}
+
 // Logarithmic mean (y - x) / log(y / x) with the series branch for
 // f2 = ((x - y) / (x + y))^2 < 1e-4, math_utils.hpp:8-24.
 DG_HD double ln_mean(double x, double y) {
@@ -187,27 +202,14 @@ DG_HD NodePrims<DIM> node_prims(double gamma, const double* u) {
   NodePrims<DIM> q;
   q.rho = u[0];
   double v2 = 0.0;
-  for (int k = 0; k < DIM; ++k) q.v[k] = u[k + 1] / u[0];
+  const double ir = recip(u[0]);
+  for (int k = 0; k < DIM; ++k) q.v[k] = u[k + 1] * ir;
   for (int k = 0; k < DIM; ++k) v2 += q.v[k] * q.v[k];
   q.p = (gamma - 1.0) * (u[DIM + 1] - 0.5 * u[0] * v2);
-  q.beta = 0.5 * u[0] / q.p;
+  q.beta = 0.5 * u[0] * recip(q.p);
   q.kin = 0.5 * v2;
-  q.c = sqrt(gamma * q.p / u[0]);
+  q.c = sqrt(gamma * q.p * ir);
   return q;
-}
-
-// 1 / x.  DGSEM_FAST_RCP (experiment): the hardware reciprocal refined by two
-// Newton steps instead of the IEEE division sequence.
-DG_HD double recip(double x) {
-#if defined(DGSEM_FAST_RCP) && defined(__HIP_DEVICE_COMPILE__)
-  double r = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-x, r, 1.0);
-  return fma(r, e, r);
-#else
-  return 1.0 / x;
-#endif
 }
 
 // Chandrashekar two-point flux from hoisted node quantities,
@@ -266,7 +268,7 @@ DG_HD void flux_chandrashekar(double inv_gm1, const NodePrims<DIM>& a, const Nod
 template <int DIM>
 DG_HD NodePrims<DIM> face_prims(double gamma, const double* u) {
   NodePrims<DIM> q;
-  const double ir = 1.0 / u[0];
+  const double ir = recip(u[0]);
   double v2 = 0.0;
   q.rho = u[0];
   for (int k = 0; k < DIM; ++k) q.v[k] = u[k + 1] * ir;
