@@ -27,7 +27,7 @@ template <int N, int BLOCK>
 constexpr int batch_lds_doubles() {
   using S = Shape<2, N>;
   constexpr int EPW = BLOCK / S::LINES;
-  return EPW * (S::NV + S::NP) * S::PN + EPW;
+  return EPW * (S::SU + S::SP) + EPW;
 }
 
 template <int N, int BLOCK, bool SC, bool NODAL, int SF, int FF>
@@ -41,8 +41,8 @@ __device__ __forceinline__ void batch_tile(const KParams& p, const int64_t e0, c
   static_assert(EPW >= 1, "workgroup too small for one element");
   static_assert(NP >= NV && NP >= 2, "s_p must be able to hold s_du and scratch");
   double* const s_u = smem;
-  double* const s_p = smem + EPW * NV * PN;
-  double* const s_alpha = s_p + EPW * NP * PN;
+  double* const s_p = smem + EPW * S::SU;
+  double* const s_alpha = s_p + EPW * S::SP;
   double* const s_du = s_p;
   // The thread's roles (line, direction, node positions) do not change from
   // tile to tile; computed from a plain threadIdx.x the compiler would hoist

@@ -41,6 +41,12 @@ struct Shape {
   // LDS planes hold one padding slot per row of n nodes, so that line
   // threads walking rows (node stride n) hit distinct banks.
   static constexpr int PN = NN + NN / n;
+  // Element slot strides of the LDS state (NV planes) and primitive (NP
+  // planes) arrays, padded to 8 doubles modulo 32 so that the slots of the
+  // elements sharing a wave start in different banks.
+  static constexpr int slot_pad(int x) { return (8 - x % 32 + 32) % 32; }
+  static constexpr int SU = NV * PN + slot_pad(NV * PN);
+  static constexpr int SP = NP * PN + slot_pad(NP * PN);
 };
 
 // Padded LDS position of a node (see Shape::PN).
@@ -94,7 +100,7 @@ __device__ void block_alpha(const KParams& p, const double* s_u, double* s_w, do
   for (int i = tid; i < EPW * NN; i += BLOCK) {
     const int slot = i / NN, node = i % NN;
     double s[NV];
-    for (int v = 0; v < NV; ++v) s[v] = s_u[(slot * NV + v) * S::PN + lds_pos<N>(node)];
+    for (int v = 0; v < NV; ++v) s[v] = s_u[slot * S::SU + v * S::PN + lds_pos<N>(node)];
     A[i] = slot < nvalid ? indicator_variable<DIM>(s) : 0.0;
   }
   __syncthreads();
@@ -203,8 +209,8 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
   // thread is done reading primitives, the per-direction sums (s_du).
   static_assert(NP >= NV && NP >= 2, "s_p must be able to hold s_du and scratch");
   constexpr int PN = S::PN;
-  __shared__ double s_u[EPW * NV * PN];
-  __shared__ double s_p[EPW * NP * PN];
+  __shared__ double s_u[EPW * S::SU];
+  __shared__ double s_p[EPW * S::SP];
   __shared__ double s_alpha[EPW];
   double* const s_du = s_p;
 
