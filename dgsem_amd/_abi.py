@@ -12,14 +12,14 @@ SPLIT_FORM, SHOCK_CAPTURING = 0, 1
 FLUX_CHANDRASHEKAR = 0
 FLUX_LLF, FLUX_HLLC = 0, 1
 GEOMETRY_NODAL, GEOMETRY_AFFINE = 0, 1
-BC_PERIODIC = 0
+BC_PERIODIC, BC_SLIP_WALL, BC_OUTFLOW, BC_DIRICHLET = 0, 1, 2, 3
 LAYOUT_RIGHT, LAYOUT_LEFT = 0, 1
 
 EXPORTED = (
     "dgsem_abi_version", "dgsem_config_default", "dgsem_create", "dgsem_destroy",
     "dgsem_last_error", "dgsem_sizes", "dgsem_set_indicator_parameters",
     "dgsem_node_coordinates", "dgsem_contravariant_vectors", "dgsem_inverse_jacobian",
-    "dgsem_pack_state", "dgsem_unpack_state", "dgsem_pack_state_layout",
+    "dgsem_set_boundary_values", "dgsem_pack_state", "dgsem_unpack_state", "dgsem_pack_state_layout",
     "dgsem_unpack_state_layout", "dgsem_rhs", "dgsem_get_alpha",
     "dgsem_ssprk3_stage", "dgsem_ssprk3_steps", "dgsem_synchronize",
     "dgsem_batch_create", "dgsem_batch_destroy", "dgsem_batch_last_error", "dgsem_batch_sizes",
@@ -39,6 +39,7 @@ class Config(C.Structure):
         ("volume_flux", C.c_int), ("surface_flux", C.c_int), ("fv_flux", C.c_int),
         ("alpha_max", C.c_double), ("alpha_min", C.c_double), ("alpha_smooth", C.c_int),
         ("positivity_floor", C.c_double), ("geometry", C.c_int),
+        ("face_boundary", C.c_int * 6), ("face_state", (C.c_double * 5) * 6),
     ]
 
 
@@ -71,6 +72,7 @@ def load(path: str | None = None):
     for name in ("dgsem_node_coordinates", "dgsem_contravariant_vectors", "dgsem_inverse_jacobian"):
         getattr(L, name).argtypes = [vp, vp]
     L.dgsem_pack_state.argtypes = [vp, vp, vp]
+    L.dgsem_set_boundary_values.argtypes = [vp, C.c_int, vp]
     L.dgsem_unpack_state.argtypes = [vp, vp, vp]
     L.dgsem_pack_state_layout.argtypes = [vp, C.c_int, vp, vp]
     L.dgsem_unpack_state_layout.argtypes = [vp, C.c_int, vp, vp]

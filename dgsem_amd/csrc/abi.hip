@@ -108,8 +108,9 @@ void device_geometry(const dgsem_handle* h, std::vector<double>& Ja, std::vector
   }
 }
 
-// (lower, upper) neighbour per direction; periodic wrap.
-std::vector<int> neighbour_table(const dgsem::Geometry& g) {
+// (lower, upper) neighbour per direction: periodic wrap, or -1 at a
+// boundary face of a non-periodic direction.
+std::vector<int> neighbour_table(const dgsem::Geometry& g, const int* periodic) {
   const int d = g.dim;
   std::vector<int> nb(g.E * 2 * d);
   for (int ie = 0; ie < g.cells[0]; ++ie)
@@ -120,7 +121,12 @@ std::vector<int> neighbour_table(const dgsem::Geometry& g) {
         for (int r = 0; r < d; ++r)
           for (int side = 0; side < 2; ++side) {
             int o[3] = {id[0], id[1], id[2]};
-            o[r] = (o[r] + (side ? 1 : -1) + g.cells[r]) % g.cells[r];
+            o[r] += side ? 1 : -1;
+            if (!periodic[r] && (o[r] < 0 || o[r] >= g.cells[r])) {
+              nb[e * 2 * d + 2 * r + side] = -1;
+              continue;
+            }
+            o[r] = (o[r] + g.cells[r]) % g.cells[r];
             nb[e * 2 * d + 2 * r + side] = (int)(((int64_t)o[0] * g.cells[1] + o[1]) * g.cells[2] + o[2]);
           }
       }
@@ -167,6 +173,29 @@ void fill_params(dgsem_handle* h) {
   p.grad = h->d_grad;
   p.grad_out = h->d_grad;
   p.visc_out = h->d_visc;
+  for (int k = 0; k < 3; ++k) p.cells[k] = c.cells[k];
+  for (int f = 0; f < 6; ++f) {
+    p.bc[f] = f < 2 * c.dim ? c.face_boundary[f] : dgsem::kBcPeriodic;
+    for (int v = 0; v < 5; ++v) p.bc_state[f][v] = c.face_state[f][v];
+    p.bc_values[f] = nullptr;
+  }
+}
+
+// Per-face boundary kinds: a face left at 0 takes boundary[its direction].
+void resolve_faces(dgsem_config& c) {
+  for (int f = 0; f < 6; ++f) {
+    if (f >= 2 * c.dim) {
+      c.face_boundary[f] = DGSEM_BC_PERIODIC;
+      continue;
+    }
+    if (c.face_boundary[f] == DGSEM_BC_PERIODIC) c.face_boundary[f] = c.boundary[f / 2];
+  }
+}
+
+bool has_boundaries(const dgsem_config& c) {
+  for (int f = 0; f < 2 * c.dim; ++f)
+    if (c.face_boundary[f] != DGSEM_BC_PERIODIC) return true;
+  return false;
 }
 
 int check_config(dgsem_handle* h, const dgsem_config& c) {
@@ -176,9 +205,19 @@ int check_config(dgsem_handle* h, const dgsem_config& c) {
     if (c.cells[k] < 1) return fail(h, DGSEM_E_INVALID, "cells must be positive");
     if (!(c.domain_right[k] > c.domain_left[k]))
       return fail(h, DGSEM_E_INVALID, "domain_right must exceed domain_left");
-    if (c.boundary[k] != DGSEM_BC_PERIODIC)
-      return fail(h, DGSEM_E_UNSUPPORTED, "only periodic boundaries are built");
   }
+  // faces resolved by resolve_faces
+  for (int k = 0; k < c.dim; ++k) {
+    const int lo = c.face_boundary[2 * k], hi = c.face_boundary[2 * k + 1];
+    for (int kind : {lo, hi})
+      if (kind < DGSEM_BC_PERIODIC || kind > DGSEM_BC_DIRICHLET)
+        return fail(h, DGSEM_E_INVALID, "unknown boundary condition");
+    if ((lo == DGSEM_BC_PERIODIC) != (hi == DGSEM_BC_PERIODIC))
+      return fail(h, DGSEM_E_INVALID, "a direction is periodic on both faces or on neither");
+  }
+  if (has_boundaries(c) && (c.dim != 2 || c.equation != DGSEM_EULER))
+    return fail(h, DGSEM_E_UNSUPPORTED,
+                "non-periodic boundaries are built for 2D compressible Euler only");
   const int64_t E = (int64_t)c.cells[0] * c.cells[1] * (c.dim == 3 ? c.cells[2] : 1);
   if (E > std::numeric_limits<int>::max() / 2) return fail(h, DGSEM_E_INVALID, "mesh too large");
   if (c.equation != DGSEM_EULER && c.equation != DGSEM_NAVIER_STOKES)
@@ -213,7 +252,8 @@ int launch(dgsem_handle* h, dgsem::KParams p) {
   } else {
     p.visc = nullptr;
   }
-  hipError_t e = dgsem::launch_rhs(h->cfg.dim, h->cfg.polydeg, sc, nodal, p, stream_of(h));
+  hipError_t e = dgsem::launch_rhs(h->cfg.dim, h->cfg.polydeg, sc, nodal, has_boundaries(h->cfg), p,
+                                   stream_of(h));
   if (e != hipSuccess) return hip_fail(h, e, "rhs kernel launch");
   return DGSEM_OK;
 }
@@ -296,6 +336,7 @@ int dgsem_create(const dgsem_config* cfg, void* stream, dgsem_handle** out) {
   if (!h) return DGSEM_E_ALLOC;
   h->cfg = *cfg;
   if (h->cfg.dim == 2) h->cfg.cells[2] = 1;
+  resolve_faces(h->cfg);
   int rc = check_config(h, h->cfg);
   if (rc != DGSEM_OK) {
     // keep the handle so the caller can read the message
@@ -307,9 +348,10 @@ int dgsem_create(const dgsem_config* cfg, void* stream, dgsem_handle** out) {
     *out = h;
     return fail(h, DGSEM_E_HIP, "no HIP device");
   }
+  int periodic[3] = {1, 1, 1};
+  for (int k = 0; k < h->cfg.dim; ++k) periodic[k] = h->cfg.face_boundary[2 * k] == DGSEM_BC_PERIODIC;
   try {
     h->basis = dgsem::make_basis(h->cfg.polydeg);
-    const int periodic[3] = {1, 1, 1};  // check_config admits periodic boundaries only
     h->geo = dgsem::make_geometry(h->basis, h->cfg.dim, h->cfg.cells, h->cfg.domain_left,
                                   h->cfg.domain_right, periodic);
   } catch (const std::exception& ex) {
@@ -325,7 +367,7 @@ int dgsem_create(const dgsem_config* cfg, void* stream, dgsem_handle** out) {
   if (h->cfg.geometry == DGSEM_GEOMETRY_NODAL && h->cfg.volume_integral == DGSEM_VOLUME_SHOCK_CAPTURING)
     sn = h->geo.subcell_normals;
   if ((rc = upload(h, &h->d_Ja, Ja)) || (rc = upload(h, &h->d_invJ, invJ)) ||
-      (rc = upload(h, &h->d_sn, sn)) || (rc = upload(h, &h->d_nbr, neighbour_table(h->geo))) ||
+      (rc = upload(h, &h->d_sn, sn)) || (rc = upload(h, &h->d_nbr, neighbour_table(h->geo, periodic))) ||
       (rc = upload(h, &h->d_Vinv, h->basis.Vinv)) ||
       (rc = upload(h, &h->d_alpha, std::vector<double>(h->E, 0.0))) ||
       (h->cfg.equation == DGSEM_NAVIER_STOKES &&
@@ -356,6 +398,15 @@ int dgsem_sizes(const dgsem_handle* h, int64_t* E, int* ndofs, int* nvars) {
   if (E) *E = h->E;
   if (ndofs) *ndofs = h->nn;
   if (nvars) *nvars = h->nvars;
+  return DGSEM_OK;
+}
+
+int dgsem_set_boundary_values(dgsem_handle* h, int face, const double* device_values) {
+  if (!h || !h->d_Ja) return DGSEM_E_INVALID;
+  if (face < 0 || face >= 2 * h->cfg.dim) return fail(h, DGSEM_E_INVALID, "face out of range");
+  if (h->cfg.face_boundary[face] != DGSEM_BC_DIRICHLET)
+    return fail(h, DGSEM_E_INVALID, "per-node states need a Dirichlet face");
+  h->base.bc_values[face] = device_values;
   return DGSEM_OK;
 }
 

@@ -205,6 +205,9 @@ int ad_prepare(dgsem_handle* h) {
     return afail(h, DGSEM_E_UNSUPPORTED, "tangent and adjoint are built for the LLF surface flux");
   if (c.positivity_floor > 0.0)
     return afail(h, DGSEM_E_UNSUPPORTED, "the positivity floor is not differentiated");
+  for (int f = 0; f < 2 * c.dim; ++f)
+    if (c.face_boundary[f] != DGSEM_BC_PERIODIC)
+      return afail(h, DGSEM_E_UNSUPPORTED, "tangent and adjoint are built for periodic meshes");
   if (c.equation == DGSEM_NAVIER_STOKES) {
     int rc;
     const size_t cell = (size_t)h->E * h->nvars * h->nn;

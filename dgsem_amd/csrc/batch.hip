@@ -209,6 +209,11 @@ int dgsem_batch_create(const dgsem_config* cfgs, int ncases, void* stream, dgsem
       return bfail(b, rc, msg);
     }
     b->cases.push_back(h);
+    // the batch kernel reads every face neighbour: periodic meshes only
+    for (int f = 0; f < 2 * h->cfg.dim; ++f)
+      if (h->cfg.face_boundary[f] != DGSEM_BC_PERIODIC)
+        return bfail(b, DGSEM_E_UNSUPPORTED,
+                     "batch cases must be periodic (case " + std::to_string(i) + ")");
     const int64_t cnt = h->E * h->nvars * h->nn;
     b->offset.push_back(b->total);
     b->count.push_back(cnt);

@@ -30,7 +30,7 @@ constexpr int batch_lds_doubles() {
   return EPW * (S::SU + S::SP) + EPW;
 }
 
-template <int N, int BLOCK, bool SC, bool NODAL, int SF, int FF>
+template <int N, int BLOCK, bool SC, bool NODAL, int SF, int FF, bool BCS = false>
 __device__ __forceinline__ void batch_tile(const KParams& p, const int64_t e0, const int nvalid,
                                            double* smem) {
   constexpr int DIM = 2;

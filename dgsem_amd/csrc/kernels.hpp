@@ -16,6 +16,14 @@ enum StageMode : int {
   kModeStage3 = 2,  // out = c1 u_base + c2 u + c3 L(u)    (parallel_ma3)
 };
 
+// Boundary condition kinds per face (values of DGSEM_BC_* in the C header).
+enum BoundaryKind : int {
+  kBcPeriodic = 0,
+  kBcSlipWall = 1,
+  kBcOutflow = 2,
+  kBcDirichlet = 3,
+};
+
 struct KParams {
   const double* u;       // stage input, [E][NV][NN]
   const double* u_base;  // base state for kModeStage3
@@ -42,11 +50,20 @@ struct KParams {
   double Ds[kMaxN1][kMaxN1];
   double Dhat[kMaxN1][kMaxN1];
   double inv_w[kMaxN1];
+  // boundary conditions (2D): element grid, kind per face (-x, +x, -y, +y;
+  // 0 = periodic, then nbr holds -1 at that face), constant Dirichlet states
+  // and optional per-node Dirichlet states
+  int cells[3];
+  int bc[6];
+  double bc_state[6][5];
+  const double* bc_values[6];
 };
 
 // Per-(dim, N) launchers, explicitly specialised in rhs_inst.hip.
+// boundaries: some face is not periodic (2D only)
 template <int DIM, int N>
-hipError_t launch_rhs_dn(bool shock_capturing, bool nodal, const KParams& p, hipStream_t stream);
+hipError_t launch_rhs_dn(bool shock_capturing, bool nodal, bool boundaries, const KParams& p,
+                         hipStream_t stream);
 template <int DIM, int N>
 hipError_t launch_parabolic_dn(bool nodal, const KParams& p, hipStream_t stream);
 
@@ -56,8 +73,8 @@ hipError_t launch_parabolic(int dim, int N, bool nodal, const KParams& p, hipStr
 
 // Launches the fused RHS / stage kernel for the configuration; returns
 // hipErrorInvalidValue when (dim, N, shock capturing, nodal) is not built.
-hipError_t launch_rhs(int dim, int N, bool shock_capturing, bool nodal, const KParams& p,
-                      hipStream_t stream);
+hipError_t launch_rhs(int dim, int N, bool shock_capturing, bool nodal, bool boundaries,
+                      const KParams& p, hipStream_t stream);
 
 // Layout conversion between the reference layout [e][dof][v] and the
 // internal layout [e][v][dof]; to_internal selects the direction.

@@ -55,25 +55,26 @@ __global__ void pack_left_kernel(const double* __restrict__ src, double* __restr
 
 }  // namespace
 
-hipError_t launch_rhs(int dim, int N, bool sc, bool nodal, const KParams& p, hipStream_t s) {
+hipError_t launch_rhs(int dim, int N, bool sc, bool nodal, bool bcs, const KParams& p,
+                      hipStream_t s) {
   if (dim == 2) {
     switch (N) {
-      case 1: return launch_rhs_dn<2, 1>(sc, nodal, p, s);
-      case 2: return launch_rhs_dn<2, 2>(sc, nodal, p, s);
-      case 3: return launch_rhs_dn<2, 3>(sc, nodal, p, s);
-      case 4: return launch_rhs_dn<2, 4>(sc, nodal, p, s);
-      case 5: return launch_rhs_dn<2, 5>(sc, nodal, p, s);
-      case 7: return launch_rhs_dn<2, 7>(sc, nodal, p, s);
+      case 1: return launch_rhs_dn<2, 1>(sc, nodal, bcs, p, s);
+      case 2: return launch_rhs_dn<2, 2>(sc, nodal, bcs, p, s);
+      case 3: return launch_rhs_dn<2, 3>(sc, nodal, bcs, p, s);
+      case 4: return launch_rhs_dn<2, 4>(sc, nodal, bcs, p, s);
+      case 5: return launch_rhs_dn<2, 5>(sc, nodal, bcs, p, s);
+      case 7: return launch_rhs_dn<2, 7>(sc, nodal, bcs, p, s);
       default: return hipErrorInvalidValue;
     }
   }
   if (dim == 3) {
     switch (N) {
-      case 2: return launch_rhs_dn<3, 2>(sc, nodal, p, s);
-      case 3: return launch_rhs_dn<3, 3>(sc, nodal, p, s);
-      case 4: return launch_rhs_dn<3, 4>(sc, nodal, p, s);
-      case 5: return launch_rhs_dn<3, 5>(sc, nodal, p, s);
-      case 7: return launch_rhs_dn<3, 7>(sc, nodal, p, s);
+      case 2: return launch_rhs_dn<3, 2>(sc, nodal, bcs, p, s);
+      case 3: return launch_rhs_dn<3, 3>(sc, nodal, bcs, p, s);
+      case 4: return launch_rhs_dn<3, 4>(sc, nodal, bcs, p, s);
+      case 5: return launch_rhs_dn<3, 5>(sc, nodal, bcs, p, s);
+      case 7: return launch_rhs_dn<3, 7>(sc, nodal, bcs, p, s);
       default: return hipErrorInvalidValue;
     }
   }

@@ -10,9 +10,9 @@
 
 namespace dgsem {
 template <>
-hipError_t launch_rhs_dn<DGSEM_INST_DIM, DGSEM_INST_N>(bool sc, bool nodal, const KParams& p,
-                                                        hipStream_t s) {
-  return detail::launch_dn<DGSEM_INST_DIM, DGSEM_INST_N>(sc, nodal, p, s);
+hipError_t launch_rhs_dn<DGSEM_INST_DIM, DGSEM_INST_N>(bool sc, bool nodal, bool bcs,
+                                                        const KParams& p, hipStream_t s) {
+  return detail::launch_dn<DGSEM_INST_DIM, DGSEM_INST_N>(sc, nodal, bcs, p, s);
 }
 template <>
 hipError_t launch_parabolic_dn<DGSEM_INST_DIM, DGSEM_INST_N>(bool nodal, const KParams& p,

@@ -194,10 +194,51 @@ constexpr bool pair_fence() {
   return N >= DGSEM_PAIR_FENCE_MIN_N;
 }
 
+// Outer state of a non-periodic face of a 2D compressible-Euler mesh, seen
+// from the line of the element that owns the face (face = 2 * dir + upper):
+//   slip wall  slip_wall_boundary.hpp:12-62: the inner state with the normal
+//              velocity reflected about the (unnormalised) normal nrm;
+//   outflow    outflow_boundary.hpp:61-105: the mean of the inner element's
+//              nodes on the line normal to the face (this thread's line, su
+//              holds the element's padded LDS state);
+//   Dirichlet  dirichlet_boundary.hpp:70-118: the face's constant state or,
+//              if set, the per-node state of boundary node `bnode`.
+template <int DIM, int N>
+__device__ __forceinline__ void boundary_state(const KParams& p, int face, int64_t bnode,
+                                               const double* su, const int* pdofs,
+                                               const double* uin, const double* nrm,
+                                               double* uout) {
+  constexpr int n = N + 1, NV = DIM + 2, PN = Shape<DIM, N>::PN;
+  static_assert(DIM == 2, "non-periodic boundaries are built for 2D meshes");
+  const int kind = p.bc[face];
+  if (kind == kBcSlipWall) {
+    const double rho = uin[0], vx = uin[1] / rho, vy = uin[2] / rho;
+    const double nn = nrm[0] * nrm[0] + nrm[1] * nrm[1];
+    const double vn = (vx * nrm[0] + vy * nrm[1]) / nn;
+    uout[0] = rho;
+    uout[1] = rho * (vx - 2.0 * vn * nrm[0]);
+    uout[2] = rho * (vy - 2.0 * vn * nrm[1]);
+    uout[3] = uin[3];
+  } else if (kind == kBcOutflow) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < n; ++k) acc += su[v * PN + pdofs[k]];
+      uout[v] = acc / (double)n;
+    }
+  } else {
+    const double* vals = p.bc_values[face];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) uout[v] = vals ? vals[bnode * NV + v] : p.bc_state[face][v];
+  }
+}
+
 // SF / FF: surface and subcell-FV flux kinds (kFluxLLF, kFluxHLLC) as
 // template parameters, so a kernel only carries the register footprint of
 // the fluxes it uses.
-template <int DIM, int N, int EPW, bool SC, bool NODAL, int SF, int FF>
+// BCS: some face of the (2D) mesh has a non-periodic boundary condition.
+template <int DIM, int N, int EPW, bool SC, bool NODAL, int SF, int FF, bool BCS>
 __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
     __attribute__((amdgpu_waves_per_eu(min_waves_per_simd<DIM, N, NODAL>())))
     rhs_kernel(const KParams p) {
@@ -222,38 +263,52 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
 #include "element_body.inc"
 }
 
-template <int DIM, int N, bool SC, bool NODAL, int SF, int FF>
+template <int DIM, int N, bool SC, bool NODAL, int SF, int FF, bool BCS>
 hipError_t launch_one(const KParams& p, hipStream_t stream) {
   constexpr int EPW = elems_per_block<DIM, N>();
   constexpr int BLOCK = EPW * Shape<DIM, N>::LINES;
   const int64_t nb = (p.E + EPW - 1) / EPW;
   if (nb <= 0) return hipSuccess;
   if (nb > INT32_MAX) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((rhs_kernel<DIM, N, EPW, SC, NODAL, SF, FF>), dim3((unsigned)nb), dim3(BLOCK),
-                     0, stream, p);
+  hipLaunchKernelGGL((rhs_kernel<DIM, N, EPW, SC, NODAL, SF, FF, BCS>), dim3((unsigned)nb),
+                     dim3(BLOCK), 0, stream, p);
   return hipGetLastError();
 }
 
-template <int DIM, int N, bool SC, bool NODAL>
+template <int DIM, int N, bool SC, bool NODAL, bool BCS>
 hipError_t launch_flux(const KParams& p, hipStream_t s) {
   const bool shllc = p.surface_flux == kFluxHLLC;
   if constexpr (SC) {
     const bool fhllc = p.fv_flux == kFluxHLLC;
     if (shllc)
-      return fhllc ? launch_one<DIM, N, SC, NODAL, kFluxHLLC, kFluxHLLC>(p, s)
-                   : launch_one<DIM, N, SC, NODAL, kFluxHLLC, kFluxLLF>(p, s);
-    return fhllc ? launch_one<DIM, N, SC, NODAL, kFluxLLF, kFluxHLLC>(p, s)
-                 : launch_one<DIM, N, SC, NODAL, kFluxLLF, kFluxLLF>(p, s);
+      return fhllc ? launch_one<DIM, N, SC, NODAL, kFluxHLLC, kFluxHLLC, BCS>(p, s)
+                   : launch_one<DIM, N, SC, NODAL, kFluxHLLC, kFluxLLF, BCS>(p, s);
+    return fhllc ? launch_one<DIM, N, SC, NODAL, kFluxLLF, kFluxHLLC, BCS>(p, s)
+                 : launch_one<DIM, N, SC, NODAL, kFluxLLF, kFluxLLF, BCS>(p, s);
   } else {
-    return shllc ? launch_one<DIM, N, SC, NODAL, kFluxHLLC, kFluxLLF>(p, s)
-                 : launch_one<DIM, N, SC, NODAL, kFluxLLF, kFluxLLF>(p, s);
+    return shllc ? launch_one<DIM, N, SC, NODAL, kFluxHLLC, kFluxLLF, BCS>(p, s)
+                 : launch_one<DIM, N, SC, NODAL, kFluxLLF, kFluxLLF, BCS>(p, s);
   }
 }
 
+template <int DIM, int N, bool BCS>
+hipError_t launch_geo(bool sc, bool nodal, const KParams& p, hipStream_t s) {
+  if (sc)
+    return nodal ? launch_flux<DIM, N, true, true, BCS>(p, s)
+                 : launch_flux<DIM, N, true, false, BCS>(p, s);
+  return nodal ? launch_flux<DIM, N, false, true, BCS>(p, s)
+               : launch_flux<DIM, N, false, false, BCS>(p, s);
+}
+
+// Non-periodic boundaries are built for 2D meshes only.
 template <int DIM, int N>
-hipError_t launch_dn(bool sc, bool nodal, const KParams& p, hipStream_t s) {
-  if (sc) return nodal ? launch_flux<DIM, N, true, true>(p, s) : launch_flux<DIM, N, true, false>(p, s);
-  return nodal ? launch_flux<DIM, N, false, true>(p, s) : launch_flux<DIM, N, false, false>(p, s);
+hipError_t launch_dn(bool sc, bool nodal, bool bcs, const KParams& p, hipStream_t s) {
+  if constexpr (DIM == 2) {
+    if (bcs) return launch_geo<DIM, N, true>(sc, nodal, p, s);
+  } else {
+    if (bcs) return hipErrorInvalidValue;
+  }
+  return launch_geo<DIM, N, false>(sc, nodal, p, s);
 }
 
 }  // namespace detail

@@ -42,6 +42,10 @@ class SolverConfig:
     alpha_smooth: bool = False
     positivity_floor: float = 0.0
     geometry: int = _abi.GEOMETRY_NODAL
+    # per-face boundary conditions (-x, +x, -y, +y[, -z, +z]); None: periodic
+    face_boundary: Sequence[int] | None = None
+    # constant Dirichlet states per face (conservative variables)
+    face_state: Sequence[Sequence[float]] | None = None
 
     def to_c(self) -> _abi.Config:
         L = _abi.load()
@@ -65,6 +69,14 @@ class SolverConfig:
         c.alpha_smooth = int(self.alpha_smooth)
         c.positivity_floor = self.positivity_floor
         c.geometry = self.geometry
+        if self.face_boundary is not None:
+            for f, kind in enumerate(self.face_boundary):
+                c.face_boundary[f] = int(kind)
+        if self.face_state is not None:
+            for f, st in enumerate(self.face_state):
+                if st is not None:
+                    for v, x in enumerate(st):
+                        c.face_state[f][v] = float(x)
         return c
 
 
@@ -121,6 +133,15 @@ class StructuredSolver:
 
     def set_indicator_parameters(self, alpha_max: float, alpha_min: float, alpha_smooth: bool) -> None:
         self._check(self._L.dgsem_set_indicator_parameters(self._h, alpha_max, alpha_min, int(alpha_smooth)))
+
+    def set_boundary_values(self, face: int, values) -> None:
+        """Per-node Dirichlet states of a face (device tensor [nodes][nvars],
+        kept referenced here), or None for the face's constant state."""
+        if not hasattr(self, "_bc_values"):
+            self._bc_values = {}
+        self._bc_values[face] = values
+        ptr = None if values is None else _ptr(values)
+        self._check(self._L.dgsem_set_boundary_values(self._h, int(face), ptr))
 
     def calc_rhs(self, u, du, t: float = 0.0) -> None:
         self._check(self._L.dgsem_rhs(self._h, _ptr(u), _ptr(du), float(t)))

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define DGSEM_ABI_VERSION 1
+#define DGSEM_ABI_VERSION 2
 
 /* error codes */
 #define DGSEM_OK 0
@@ -53,8 +53,12 @@ extern "C" {
 #define DGSEM_GEOMETRY_AFFINE 1 /* exact per-element metric terms of the
                                    linear mapping; 10 doubles per element */
 
-/* boundary conditions (src/boundary_condition) */
-#define DGSEM_BC_PERIODIC 0
+/* boundary conditions (src/boundary_condition); all but periodic are built
+ * for 2D compressible Euler, as in the reference */
+#define DGSEM_BC_PERIODIC 0  /* PeriodicBC, periodic_boundary.hpp:11-21 */
+#define DGSEM_BC_SLIP_WALL 1 /* SlipWallBC, slip_wall_boundary.hpp:10-66 */
+#define DGSEM_BC_OUTFLOW 2   /* OutflowBC, outflow_boundary.hpp:61-105 */
+#define DGSEM_BC_DIRICHLET 3 /* DirichletBC, dirichlet_boundary.hpp:70-118 */
 
 /* Solver configuration: the reference's template arguments
  * StructuredSolver<Equations, Basis, VolumeFlux, SurfaceFlux, Mesh,
@@ -80,6 +84,14 @@ typedef struct dgsem_config {
   int alpha_smooth;        /* stored, unused (as in the reference) */
   double positivity_floor; /* <= 0 disables; applied after stage updates */
   int geometry;            /* DGSEM_GEOMETRY_* */
+  /* Per-face boundary conditions, faces -x, +x, -y, +y, -z, +z (the
+   * reference's BoundarySet order, boundary_set.hpp:88-111).  All zero: every
+   * face takes boundary[dim].  A direction is periodic on both faces or on
+   * neither. */
+  int face_boundary[6];
+  /* DGSEM_BC_DIRICHLET outer state per face (conservative variables), used
+   * unless per-node states are set with dgsem_set_boundary_values. */
+  double face_state[6][5];
 } dgsem_config;
 
 typedef struct dgsem_handle dgsem_handle;
@@ -131,6 +143,15 @@ int dgsem_pack_state_layout(dgsem_handle* h, int layout, const double* ref_layou
                             double* internal);
 int dgsem_unpack_state_layout(dgsem_handle* h, int layout, const double* internal,
                               double* ref_layout);
+
+/* Per-node outer states of a DGSEM_BC_DIRICHLET face, for Dirichlet data
+ * that vary along the face or in time (DirichletBC with a function of
+ * (x, t), dirichlet_boundary.hpp:80-90): a device array of
+ * [(elements along the face) * (N+1)][nvars] doubles, node (element index
+ * along the face) * (N+1) + (node index along the face), read by every later
+ * RHS or stage (the caller may rewrite it between stages).  NULL reverts to
+ * the face's constant state. */
+int dgsem_set_boundary_values(dgsem_handle* h, int face, const double* device_values);
 
 /* du = L(u): StructuredSolver::calc_rhs(sol, t), structured_solver.hpp:463-480
  * (volume integral, interface flux, boundary conditions, parabolic terms,

@@ -37,6 +37,9 @@ struct orc_case {
   int64_t* nbr;
   double *sflux, *grad, *gflux, *vflux, *vsflux; /* scratch */
   double *tmp1, *tmp2, *du;                    /* SSPRK3 scratch */
+  int bc[6];                                   /* per face: ORC_BC_*, -x,+x,-y,+y,-z,+z */
+  double bc_state[6][5];                       /* constant Dirichlet states */
+  const double* bc_values[6];                  /* per-node Dirichlet states, or NULL */
 };
 
 /* ------------------------------------------------------------------------ */
@@ -1240,6 +1243,78 @@ void orc_interface_flux(orc_case* c, const double* u, double* sflux) {
   interface_flux_generic(c, u, sflux, c->surface_flux);
 }
 
+/* Boundary-face fluxes of a 2D mesh (build of the reference's BoundarySet::
+ * apply, boundary_set.hpp:88-111, for 2D compressible Euler):
+ *   slip wall   slip_wall_boundary.hpp:12-62  (outer state = inner state with
+ *               the normal velocity reflected),
+ *   outflow     outflow_boundary.hpp:61-105   (outer state = mean of the
+ *               inner element's nodes on the line normal to the face),
+ *   Dirichlet   dirichlet_boundary.hpp:70-118 (outer state given).
+ * Node order and normals follow utils/boundary_helper.hpp:69-150: the inner
+ * element's sign(J) Ja^dim at its boundary node, the inner state on the left
+ * of the flux on the upper faces and on the right on the lower faces.
+ * Per-node Dirichlet states are indexed by (element position along the face)
+ * * (N+1) + (node position along the face). */
+static void boundary_fluxes(orc_case* c, const double* u, double* sflux) {
+  if (c->dim != 2) return;
+  const int n = c->n, nv = c->nvars, fn = face_nodes(c);
+  for (int face = 0; face < 4; ++face) {
+    const int kind = c->bc[face];
+    if (kind == ORC_BC_PERIODIC) continue;
+    const int dim = face / 2, upper = face % 2;
+    const int count = dim == 0 ? c->ny : c->nx;
+    for (int idx = 0; idx < count; ++idx) {
+      const int ie = dim == 0 ? (upper ? c->nx - 1 : 0) : idx;
+      const int je = dim == 0 ? idx : (upper ? c->ny - 1 : 0);
+      const int64_t e = (int64_t)ie * c->ny + je;
+      for (int q = 0; q < n; ++q) {
+        /* boundary node and the line through it, normal to the face */
+        const int bi = dim == 0 ? (upper ? n - 1 : 0) : q;
+        const int bj = dim == 0 ? q : (upper ? n - 1 : 0);
+        const int bdof = bj * n + bi;
+        double uin[4], uout[4], nrm[2], f[4];
+        for (int v = 0; v < nv; ++v) uin[v] = U(u, e, bdof, v);
+        const double sgn = INVJ(e, bdof) >= 0.0 ? 1.0 : -1.0;
+        for (int comp = 0; comp < 2; ++comp) nrm[comp] = sgn * JA(e, bdof, dim, comp);
+        if (kind == ORC_BC_SLIP_WALL) {
+          const double rho = uin[0], vx = uin[1] / rho, vy = uin[2] / rho;
+          const double nn = nrm[0] * nrm[0] + nrm[1] * nrm[1];
+          const double vn = (vx * nrm[0] + vy * nrm[1]) / nn;
+          uout[0] = rho;
+          uout[1] = rho * (vx - 2.0 * vn * nrm[0]);
+          uout[2] = rho * (vy - 2.0 * vn * nrm[1]);
+          uout[3] = uin[3];
+        } else if (kind == ORC_BC_OUTFLOW) {
+          for (int v = 0; v < nv; ++v) {
+            double acc = 0.0;
+            for (int k = 0; k < n; ++k) {
+              const int dof = dim == 0 ? bj * n + k : k * n + bi;
+              acc += U(u, e, dof, v);
+            }
+            uout[v] = acc / (double)n;
+          }
+        } else {
+          const int bnode = idx * n + q;
+          for (int v = 0; v < nv; ++v)
+            uout[v] = c->bc_values[face] ? c->bc_values[face][bnode * nv + v] : c->bc_state[face][v];
+        }
+        if (upper)
+          surface_flux_fn(c, c->surface_flux, uin, uout, nrm, f);
+        else
+          surface_flux_fn(c, c->surface_flux, uout, uin, nrm, f);
+        for (int v = 0; v < nv; ++v) SF(sflux, e, face * fn + q, v) = f[v];
+      }
+    }
+  }
+}
+
+void orc_set_boundary(orc_case* c, int face, int kind, const double* state,
+                      const double* values) {
+  c->bc[face] = kind;
+  for (int v = 0; v < 5; ++v) c->bc_state[face][v] = state ? state[v] : 0.0;
+  c->bc_values[face] = values;
+}
+
 /* SurfaceIntegral<.,2/3>::integral, surface_integral.hpp:35-80,82-154 */
 static void surface_integral_generic(orc_case* c, const double* sflux,
                                      double* du, int nv) {
@@ -1571,6 +1646,7 @@ void orc_rhs(orc_case* c, const double* u, double* du, double* alpha) {
   orc_volume_integral(c, u, du, alpha);
   orc_interface_flux(c, u, c->sflux);
   /* periodic boundary conditions are no-ops (periodic_boundary.hpp:11-21) */
+  boundary_fluxes(c, u, c->sflux);
   if (c->eq == ORC_NAVIER_STOKES) viscous_terms(c, u, du);
   orc_surface_integral(c, c->sflux, du);
   orc_apply_jacobian(c, du);

@@ -87,6 +87,17 @@ void orc_apply_jacobian(orc_case* c, double* du);
 /* BR1 gradient (E, nn, nvars, dim) and the full parabolic contribution. */
 void orc_gradients(orc_case* c, const double* u, double* gradient);
 
+/* Boundary condition of one face of a 2D mesh (face = 2*dim + upper); the
+ * mesh must be created non-periodic in that direction.  state: constant
+ * Dirichlet state (conservative); values: per-node Dirichlet states
+ * [face nodes][nvars] (kept by pointer), or NULL. */
+#define ORC_BC_PERIODIC 0
+#define ORC_BC_SLIP_WALL 1
+#define ORC_BC_OUTFLOW 2
+#define ORC_BC_DIRICHLET 3
+void orc_set_boundary(orc_case* c, int face, int kind, const double* state,
+                      const double* values);
+
 /* SSPRK3::step (SSPRK3.hpp:159-181), nsteps times with fixed dt. */
 void orc_ssprk3(orc_case* c, double* u, double dt, int nsteps);
 

@@ -22,6 +22,7 @@ _lib = None
 EULER, NAVIER_STOKES = 0, 1
 SPLIT_FORM, SHOCK_CAPTURING = 0, 1
 FLUX_LLF, FLUX_HLLC = 0, 1
+BC_PERIODIC, BC_SLIP_WALL, BC_OUTFLOW, BC_DIRICHLET = 0, 1, 2, 3
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
@@ -77,6 +78,7 @@ def lib():
     L.orc_physical_flux.argtypes = [C.c_int, C.c_double, _dp, _dp, _dp]
     L.orc_kinetic_energy.restype = C.c_double
     L.orc_kinetic_energy.argtypes = [vp, _dp]
+    L.orc_set_boundary.argtypes = [vp, C.c_int, C.c_int, vp, vp]
     L.orc_analyze.argtypes = [vp, _dp, _dp]
     _lib = L
     return L
@@ -250,6 +252,20 @@ class Case:
 
     def kinetic_energy(self, u):
         return lib().orc_kinetic_energy(self._h, np.ascontiguousarray(u, dtype=np.float64))
+
+    def set_boundary(self, face: int, kind: int, state=None, values=None):
+        """Boundary condition of one face of a 2D mesh (face = 2*dim + upper);
+        values: per-node Dirichlet states [face nodes][nvars] (kept alive here)."""
+        st = None
+        if state is not None:  # the C side reads five values
+            st = np.zeros(5)
+            st[:len(state)] = np.asarray(state, dtype=np.float64)
+        va = None if values is None else np.ascontiguousarray(values, dtype=np.float64)
+        if not hasattr(self, "_bc_keep"):
+            self._bc_keep = {}
+        self._bc_keep[face] = (st, va)
+        lib().orc_set_boundary(self._h, face, kind, None if st is None else st.ctypes.data,
+                               None if va is None else va.ctypes.data)
 
     def analyze(self, u):
         out = np.zeros(5)

@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_defaults_without_gpu():
     lib = _abi.load()
-    assert lib.dgsem_abi_version() == 1
+    assert lib.dgsem_abi_version() == 2
     c = _abi.Config()
     lib.dgsem_config_default(ctypes.byref(c), 3, 3)
     assert (c.dim, c.polydeg, c.gamma, c.alpha_max, c.alpha_min) == (3, 3, 1.4, 0.5, 0.001)
@@ -48,4 +48,35 @@ def test_invalid_configuration_is_rejected_with_a_message():
     rc = lib.dgsem_create(ctypes.byref(c), None, ctypes.byref(h))
     assert rc == -1
     assert b"dim" in lib.dgsem_last_error(h)
+    lib.dgsem_destroy(h)
+
+
+@pytest.mark.parametrize("dim,faces,rc,word", [
+    (2, [1, 0, 0, 0], -1, b"periodic on both faces"),   # -x wall, +x periodic
+    (2, [5, 5, 0, 0], -1, b"unknown boundary"),
+    (3, [1, 1, 0, 0, 0, 0], -2, b"2D compressible Euler"),
+])
+def test_boundary_configuration_errors_without_gpu(dim, faces, rc, word):
+    """Face kinds are checked before any device work: a direction must be
+    periodic on both faces or on neither, kinds must be known and
+    non-periodic faces are built for 2D Euler only."""
+    lib = _abi.load()
+    c = _abi.Config()
+    lib.dgsem_config_default(ctypes.byref(c), dim, 3)
+    for f, k in enumerate(faces):
+        c.face_boundary[f] = k
+    h = ctypes.c_void_p()
+    assert lib.dgsem_create(ctypes.byref(c), None, ctypes.byref(h)) == rc
+    assert word in lib.dgsem_last_error(h)
+    lib.dgsem_destroy(h)
+
+
+def test_navier_stokes_walls_not_built_without_gpu():
+    lib = _abi.load()
+    c = _abi.Config()
+    lib.dgsem_config_default(ctypes.byref(c), 2, 3)
+    c.equation = _abi.NAVIER_STOKES
+    c.boundary[0] = _abi.BC_SLIP_WALL  # both x faces through the per-direction kind
+    h = ctypes.c_void_p()
+    assert lib.dgsem_create(ctypes.byref(c), None, ctypes.byref(h)) == -2
     lib.dgsem_destroy(h)
