@@ -13,6 +13,7 @@ FLUX_CHANDRASHEKAR = 0
 FLUX_LLF, FLUX_HLLC = 0, 1
 GEOMETRY_NODAL, GEOMETRY_AFFINE = 0, 1
 BC_PERIODIC, BC_SLIP_WALL, BC_OUTFLOW, BC_DIRICHLET = 0, 1, 2, 3
+BC_NS_NOSLIP_WALL, BC_NS_SLIP_WALL, BC_MIXED_DIRICHLET_SLIP_WALL = 4, 5, 6
 LAYOUT_RIGHT, LAYOUT_LEFT = 0, 1
 
 EXPORTED = (

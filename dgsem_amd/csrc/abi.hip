@@ -210,14 +210,16 @@ int check_config(dgsem_handle* h, const dgsem_config& c) {
   for (int k = 0; k < c.dim; ++k) {
     const int lo = c.face_boundary[2 * k], hi = c.face_boundary[2 * k + 1];
     for (int kind : {lo, hi})
-      if (kind < DGSEM_BC_PERIODIC || kind > DGSEM_BC_DIRICHLET)
+      if (kind < DGSEM_BC_PERIODIC || kind > DGSEM_BC_MIXED_DIRICHLET_SLIP_WALL)
         return fail(h, DGSEM_E_INVALID, "unknown boundary condition");
+      else if ((kind == DGSEM_BC_NS_NOSLIP_WALL || kind == DGSEM_BC_NS_SLIP_WALL) &&
+               c.equation != DGSEM_NAVIER_STOKES)
+        return fail(h, DGSEM_E_INVALID, "the adiabatic walls are Navier-Stokes boundary conditions");
     if ((lo == DGSEM_BC_PERIODIC) != (hi == DGSEM_BC_PERIODIC))
       return fail(h, DGSEM_E_INVALID, "a direction is periodic on both faces or on neither");
   }
-  if (has_boundaries(c) && (c.dim != 2 || c.equation != DGSEM_EULER))
-    return fail(h, DGSEM_E_UNSUPPORTED,
-                "non-periodic boundaries are built for 2D compressible Euler only");
+  if (has_boundaries(c) && c.dim != 2)
+    return fail(h, DGSEM_E_UNSUPPORTED, "non-periodic boundaries are built for 2D meshes only");
   const int64_t E = (int64_t)c.cells[0] * c.cells[1] * (c.dim == 3 ? c.cells[2] : 1);
   if (E > std::numeric_limits<int>::max() / 2) return fail(h, DGSEM_E_INVALID, "mesh too large");
   if (c.equation != DGSEM_EULER && c.equation != DGSEM_NAVIER_STOKES)
@@ -241,12 +243,16 @@ int check_config(dgsem_handle* h, const dgsem_config& c) {
 hipStream_t stream_of(const dgsem_handle* h) { return h->stream; }
 
 int launch(dgsem_handle* h, dgsem::KParams p) {
+  for (int f = 0; f < 2 * h->cfg.dim; ++f)
+    if (p.bc[f] == DGSEM_BC_MIXED_DIRICHLET_SLIP_WALL && !p.bc_values[f])
+      return fail(h, DGSEM_E_INVALID, "a mixed Dirichlet/slip-wall face needs its per-node states");
   const bool sc = h->cfg.volume_integral == DGSEM_VOLUME_SHOCK_CAPTURING;
   const bool nodal = h->cfg.geometry == DGSEM_GEOMETRY_NODAL;
   if (h->cfg.equation == DGSEM_NAVIER_STOKES) {
     // BR1 gradient and viscous part of the stage input, then the inviscid
     // kernel adds the viscous part before -1/J
-    hipError_t e = dgsem::launch_parabolic(h->cfg.dim, h->cfg.polydeg, nodal, p, stream_of(h));
+    hipError_t e = dgsem::launch_parabolic(h->cfg.dim, h->cfg.polydeg, nodal, has_boundaries(h->cfg),
+                                           p, stream_of(h));
     if (e != hipSuccess) return hip_fail(h, e, "parabolic kernel launch");
     p.visc = h->d_visc;
   } else {
@@ -404,8 +410,9 @@ int dgsem_sizes(const dgsem_handle* h, int64_t* E, int* ndofs, int* nvars) {
 int dgsem_set_boundary_values(dgsem_handle* h, int face, const double* device_values) {
   if (!h || !h->d_Ja) return DGSEM_E_INVALID;
   if (face < 0 || face >= 2 * h->cfg.dim) return fail(h, DGSEM_E_INVALID, "face out of range");
-  if (h->cfg.face_boundary[face] != DGSEM_BC_DIRICHLET)
-    return fail(h, DGSEM_E_INVALID, "per-node states need a Dirichlet face");
+  const int kind = h->cfg.face_boundary[face];
+  if (kind != DGSEM_BC_DIRICHLET && kind != DGSEM_BC_MIXED_DIRICHLET_SLIP_WALL)
+    return fail(h, DGSEM_E_INVALID, "per-node states need a Dirichlet or mixed face");
   h->base.bc_values[face] = device_values;
   return DGSEM_OK;
 }

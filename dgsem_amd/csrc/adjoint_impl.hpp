@@ -725,7 +725,7 @@ hipError_t launch_gradient_t(const KParams& p, hipStream_t s) {
   constexpr int BLOCK = EPW * Shape<DIM, N>::LINES;
   const int64_t nb = (p.E + EPW - 1) / EPW;
   if (nb <= 0) return hipSuccess;
-  hipLaunchKernelGGL((gradient_kernel<DIM, N, EPW, NODAL>), dim3((unsigned)nb), dim3(BLOCK), 0, s, p);
+  hipLaunchKernelGGL((gradient_kernel<DIM, N, EPW, NODAL, false>), dim3((unsigned)nb), dim3(BLOCK), 0, s, p);
   return hipGetLastError();
 }
 

@@ -22,6 +22,9 @@ enum BoundaryKind : int {
   kBcSlipWall = 1,
   kBcOutflow = 2,
   kBcDirichlet = 3,
+  kBcNsNoSlipWall = 4,  // Navier-Stokes adiabatic walls
+  kBcNsSlipWall = 5,
+  kBcMixedDirichletSlipWall = 6,  // per-node choice, states [node][NV + 1]
 };
 
 struct KParams {
@@ -65,11 +68,12 @@ template <int DIM, int N>
 hipError_t launch_rhs_dn(bool shock_capturing, bool nodal, bool boundaries, const KParams& p,
                          hipStream_t stream);
 template <int DIM, int N>
-hipError_t launch_parabolic_dn(bool nodal, const KParams& p, hipStream_t stream);
+hipError_t launch_parabolic_dn(bool nodal, bool boundaries, const KParams& p, hipStream_t stream);
 
 // Gradient and viscous kernels of the Navier-Stokes terms: fill p.grad_out
 // (with p.grad == p.grad_out) and p.visc_out from p.u.
-hipError_t launch_parabolic(int dim, int N, bool nodal, const KParams& p, hipStream_t stream);
+hipError_t launch_parabolic(int dim, int N, bool nodal, bool boundaries, const KParams& p,
+                            hipStream_t stream);
 
 // Launches the fused RHS / stage kernel for the configuration; returns
 // hipErrorInvalidValue when (dim, N, shock capturing, nodal) is not built.

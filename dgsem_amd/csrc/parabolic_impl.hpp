@@ -15,6 +15,9 @@
 //     (parabolic_viscous.hpp:318-334 3D, :570-585 2D), lifted with l_hat.
 //     The result (before -1/J) is added by the inviscid kernel.
 // Layouts: gradient [E][dim][var][dof]; viscous part [E][var][dof].
+// BCS (2D): some face is not periodic; the neighbour table holds -1 there and
+// the face values come from the boundary condition (wall_gradient_value and
+// the viscous kernel's wall branch).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -86,7 +89,38 @@ __device__ __forceinline__ double inv_jac(const KParams& p, int64_t e, int dof) 
     return p.invJ[e];
 }
 
-template <int DIM, int N, int EPW, bool NODAL>
+// Gradient face value at a boundary node: the adiabatic walls take the inner
+// gradient variables q with the velocity set to zero (no slip,
+// navier_stokes_wall_boundary.hpp:81-114) or with its component along the
+// normal nrm = sign(J) Ja^dir removed (slip, :163-197 and :28-36).
+// Dirichlet faces take the outer state's variables (computed by the caller,
+// dirichlet_boundary.hpp:122-162).  Kinds without a gradient hook
+// (boundary_set.hpp:16-33) leave the face value at zero, as the reference's
+// cleared face array does.
+template <int DIM>
+__device__ __forceinline__ void wall_gradient_value(int kind, const double* q, const double* nrm,
+                                                    double* qw) {
+  constexpr int NV = DIM + 2;
+  static_assert(DIM == 2, "non-periodic boundaries are built for 2D meshes");
+  if (kind == kBcNsNoSlipWall) {
+    qw[0] = q[0];
+    qw[1] = 0.0;
+    qw[2] = 0.0;
+    qw[3] = q[3];
+  } else if (kind == kBcNsSlipWall) {
+    const double nn = nrm[0] * nrm[0] + nrm[1] * nrm[1];
+    const double vn = (q[1] * nrm[0] + q[2] * nrm[1]) / nn;
+    qw[0] = q[0];
+    qw[1] = q[1] - vn * nrm[0];
+    qw[2] = q[2] - vn * nrm[1];
+    qw[3] = q[3];
+  } else {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) qw[v] = 0.0;
+  }
+}
+
+template <int DIM, int N, int EPW, bool NODAL, bool BCS>
 __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) gradient_kernel(const KParams p) {
   using S = Shape<DIM, N>;
   constexpr int n = S::n, NV = S::NV, NN = S::NN, FN = S::FN, L = S::LINES;
@@ -129,6 +163,32 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) gradient_kernel(
 #pragma unroll 1
     for (int side = 0; side < 2; ++side) {
       const int64_t nb = p.nbr[e * 2 * DIM + 2 * dir + side];
+      if constexpr (BCS) {
+        if (nb < 0) {
+          const int dof_own = side == 0 ? dofs[0] : dofs[n - 1];
+          const double sg = inv_jac<NODAL, NN>(p, e, dof_own) >= 0.0 ? 1.0 : -1.0;
+          double qo[NV], nrm[DIM], qw[NV];
+          const int face = 2 * dir + side;
+          if (p.bc[face] == kBcDirichlet) {
+            const int64_t along = dir == 0 ? e % p.cells[1] : e / p.cells[1];
+            const int64_t bnode = along * n + line;
+            const double* vals = p.bc_values[face];
+            double so[NV];
+            for (int v = 0; v < NV; ++v) so[v] = vals ? vals[bnode * NV + v] : p.bc_state[face][v];
+            grad_vars<DIM>(p.gamma, so, qw);
+          } else {
+            for (int v = 0; v < NV; ++v) qo[v] = q[v * NN + dof_own];
+            for (int c = 0; c < DIM; ++c) nrm[c] = sg * metric<NODAL, DIM, NN>(p, e, dir, c, dof_own);
+            wall_gradient_value<DIM>(p.bc[face], qo, nrm, qw);
+          }
+          if (side == 0) {
+            for (int v = 0; v < NV; ++v) acc[0][v] -= sg * p.lhat0 * qw[v];
+          } else {
+            for (int v = 0; v < NV; ++v) acc[n - 1][v] += sg * p.lhatN * qw[v];
+          }
+          continue;
+        }
+      }
       const int dof_nb = side == 0 ? dofs[n - 1] : dofs[0];
       double s[NV], qn[NV];
       for (int v = 0; v < NV; ++v) s[v] = p.u[(nb * NV + v) * NN + dof_nb];
@@ -187,7 +247,7 @@ __device__ __forceinline__ void contravariant_viscous(const KParams& p, int64_t 
   }
 }
 
-template <int DIM, int N, int EPW, bool NODAL>
+template <int DIM, int N, int EPW, bool NODAL, bool BCS>
 __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) viscous_kernel(const KParams p) {
   using S = Shape<DIM, N>;
   constexpr int n = S::n, NV = S::NV, NN = S::NN, FN = S::FN, L = S::LINES;
@@ -229,13 +289,36 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) viscous_kernel(c
         for (int i = 0; i < n; ++i) r += p.Dhat[k][i] * f[v * NN + dofs[i]];
         acc[k][v] = r;
       }
+    // Boundary faces (BCS): the own node's (-F_v).(sign(J) Ja^dir) for the
+    // momentum components only at a no-slip adiabatic wall
+    // (navier_stokes_wall_boundary.hpp:116-145), for every variable at a
+    // Dirichlet face (dirichlet_boundary.hpp:164-194); every other kind
+    // leaves the face value at zero (:199-216, boundary_set.hpp:35-51).
+    bool bnd_lo = false, bnd_hi = false;
+    int vlo[2] = {0, 0}, vhi[2] = {0, 0};  // variables carried, per side
+    if constexpr (BCS) {
+      bnd_lo = p.nbr[e * 2 * DIM + 2 * dir] < 0;
+      bnd_hi = p.nbr[e * 2 * DIM + 2 * dir + 1] < 0;
+      for (int side = 0; side < 2; ++side) {
+        const int kind = p.bc[2 * dir + side];
+        vlo[side] = kind == kBcDirichlet ? 0 : 1;
+        vhi[side] = kind == kBcDirichlet ? NV : (kind == kBcNsNoSlipWall ? NV - 1 : 0);
+      }
+    }
     // lower face: this element is primary, (-F_v).(sign(J) Ja^dir) at its node
     {
       const double sg = inv_jac<NODAL, NN>(p, e, dofs[0]) >= 0.0 ? 1.0 : -1.0;
-      for (int v = 0; v < NV; ++v) acc[0][v] -= sg * f[v * NN + dofs[0]] * p.lhat0;
+      if (!bnd_lo) {
+        for (int v = 0; v < NV; ++v) acc[0][v] -= sg * f[v * NN + dofs[0]] * p.lhat0;
+      } else {
+        for (int v = vlo[0]; v < vhi[0]; ++v) acc[0][v] -= sg * f[v * NN + dofs[0]] * p.lhat0;
+      }
     }
-    // upper face: the upper neighbour is primary
-    {
+    if (bnd_hi) {
+      const double sg = inv_jac<NODAL, NN>(p, e, dofs[n - 1]) >= 0.0 ? 1.0 : -1.0;
+      for (int v = vlo[1]; v < vhi[1]; ++v) acc[n - 1][v] += sg * f[v * NN + dofs[n - 1]] * p.lhatN;
+    } else {
+      // upper face: the upper neighbour is primary
       const int64_t nb = p.nbr[e * 2 * DIM + 2 * dir + 1];
       double s[NV], q[NV], g[DIM][NV], fv[DIM][NV];
       for (int v = 0; v < NV; ++v) s[v] = p.u[(nb * NV + v) * NN + dofs[0]];
@@ -284,16 +367,16 @@ constexpr int parabolic_elems_per_block() {
   return e < 1 ? 1 : e;
 }
 
-template <int DIM, int N, bool NODAL>
+template <int DIM, int N, bool NODAL, bool BCS>
 hipError_t launch_parabolic_t(const KParams& p, hipStream_t s) {
   constexpr int EPW = parabolic_elems_per_block<DIM, N>();
   constexpr int BLOCK = EPW * Shape<DIM, N>::LINES;
   const int64_t nb = (p.E + EPW - 1) / EPW;
   if (nb <= 0) return hipSuccess;
-  hipLaunchKernelGGL((gradient_kernel<DIM, N, EPW, NODAL>), dim3((unsigned)nb), dim3(BLOCK), 0, s, p);
+  hipLaunchKernelGGL((gradient_kernel<DIM, N, EPW, NODAL, BCS>), dim3((unsigned)nb), dim3(BLOCK), 0, s, p);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
-  hipLaunchKernelGGL((viscous_kernel<DIM, N, EPW, NODAL>), dim3((unsigned)nb), dim3(BLOCK), 0, s, p);
+  hipLaunchKernelGGL((viscous_kernel<DIM, N, EPW, NODAL, BCS>), dim3((unsigned)nb), dim3(BLOCK), 0, s, p);
   return hipGetLastError();
 }
 

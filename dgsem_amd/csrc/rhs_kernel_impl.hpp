@@ -197,12 +197,18 @@ constexpr bool pair_fence() {
 // Outer state of a non-periodic face of a 2D compressible-Euler mesh, seen
 // from the line of the element that owns the face (face = 2 * dir + upper):
 //   slip wall  slip_wall_boundary.hpp:12-62: the inner state with the normal
-//              velocity reflected about the (unnormalised) normal nrm;
+//              velocity reflected about the (unnormalised) normal nrm; the
+//              Navier-Stokes adiabatic walls use the same reflection for
+//              their inviscid flux (navier_stokes_wall_boundary.hpp:12-27,
+//              :41-77, :150-161);
 //   outflow    outflow_boundary.hpp:61-105: the mean of the inner element's
 //              nodes on the line normal to the face (this thread's line, su
 //              holds the element's padded LDS state);
 //   Dirichlet  dirichlet_boundary.hpp:70-118: the face's constant state or,
-//              if set, the per-node state of boundary node `bnode`.
+//              if set, the per-node state of boundary node `bnode`;
+//   mixed      dirichlet_boundary.hpp:198-275: per node either the given
+//              state or the slip-wall reflection, chosen by the node's flag
+//              (the last of its NV + 1 values).
 template <int DIM, int N>
 __device__ __forceinline__ void boundary_state(const KParams& p, int face, int64_t bnode,
                                                const double* su, const int* pdofs,
@@ -211,7 +217,13 @@ __device__ __forceinline__ void boundary_state(const KParams& p, int face, int64
   constexpr int n = N + 1, NV = DIM + 2, PN = Shape<DIM, N>::PN;
   static_assert(DIM == 2, "non-periodic boundaries are built for 2D meshes");
   const int kind = p.bc[face];
-  if (kind == kBcSlipWall) {
+  const double* vals = p.bc_values[face];
+  const bool mixed_dirichlet = kind == kBcMixedDirichletSlipWall && vals[bnode * (NV + 1) + NV] != 0.0;
+  if (mixed_dirichlet) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) uout[v] = vals[bnode * (NV + 1) + v];
+  } else if (kind == kBcSlipWall || kind == kBcNsNoSlipWall || kind == kBcNsSlipWall ||
+             kind == kBcMixedDirichletSlipWall) {
     const double rho = uin[0], vx = uin[1] / rho, vy = uin[2] / rho;
     const double nn = nrm[0] * nrm[0] + nrm[1] * nrm[1];
     const double vn = (vx * nrm[0] + vy * nrm[1]) / nn;
@@ -228,7 +240,6 @@ __device__ __forceinline__ void boundary_state(const KParams& p, int face, int64
       uout[v] = acc / (double)n;
     }
   } else {
-    const double* vals = p.bc_values[face];
 #pragma unroll
     for (int v = 0; v < NV; ++v) uout[v] = vals ? vals[bnode * NV + v] : p.bc_state[face][v];
   }

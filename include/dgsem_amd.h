@@ -54,11 +54,21 @@ extern "C" {
                                    linear mapping; 10 doubles per element */
 
 /* boundary conditions (src/boundary_condition); all but periodic are built
- * for 2D compressible Euler, as in the reference */
+ * for 2D meshes (the reference specialises them for 1D and 2D) */
 #define DGSEM_BC_PERIODIC 0  /* PeriodicBC, periodic_boundary.hpp:11-21 */
 #define DGSEM_BC_SLIP_WALL 1 /* SlipWallBC, slip_wall_boundary.hpp:10-66 */
 #define DGSEM_BC_OUTFLOW 2   /* OutflowBC, outflow_boundary.hpp:61-105 */
 #define DGSEM_BC_DIRICHLET 3 /* DirichletBC, dirichlet_boundary.hpp:70-118 */
+/* Navier-Stokes adiabatic walls: NSNoSlipAdiabaticWallBC
+ * (navier_stokes_wall_boundary.hpp:39-146) and NSSlipAdiabaticWallBC
+ * (:148-217).  Under Navier-Stokes the other kinds have no gradient or
+ * viscous face values, which stay zero as in the reference. */
+#define DGSEM_BC_NS_NOSLIP_WALL 4
+#define DGSEM_BC_NS_SLIP_WALL 5
+/* MixedDirichletSlipWallBC (dirichlet_boundary.hpp:198-275): per boundary
+ * node either a Dirichlet state or the slip-wall reflection.  The node's
+ * state and choice come from dgsem_set_boundary_values (required). */
+#define DGSEM_BC_MIXED_DIRICHLET_SLIP_WALL 6
 
 /* Solver configuration: the reference's template arguments
  * StructuredSolver<Equations, Basis, VolumeFlux, SurfaceFlux, Mesh,
@@ -150,7 +160,10 @@ int dgsem_unpack_state_layout(dgsem_handle* h, int layout, const double* interna
  * [(elements along the face) * (N+1)][nvars] doubles, node (element index
  * along the face) * (N+1) + (node index along the face), read by every later
  * RHS or stage (the caller may rewrite it between stages).  NULL reverts to
- * the face's constant state. */
+ * the face's constant state.  For a DGSEM_BC_MIXED_DIRICHLET_SLIP_WALL face
+ * each node holds nvars + 1 doubles: the state, then a flag that is nonzero
+ * where the node takes the state (the reference's use_dirichlet(x, t)) and
+ * zero where it is a slip wall. */
 int dgsem_set_boundary_values(dgsem_handle* h, int face, const double* device_values);
 
 /* du = L(u): StructuredSolver::calc_rhs(sol, t), structured_solver.hpp:463-480

@@ -1244,7 +1244,7 @@ void orc_interface_flux(orc_case* c, const double* u, double* sflux) {
 }
 
 /* Boundary-face fluxes of a 2D mesh (build of the reference's BoundarySet::
- * apply, boundary_set.hpp:88-111, for 2D compressible Euler):
+ * apply, boundary_set.hpp:88-111, for 2D compressible flow):
  *   slip wall   slip_wall_boundary.hpp:12-62  (outer state = inner state with
  *               the normal velocity reflected),
  *   outflow     outflow_boundary.hpp:61-105   (outer state = mean of the
@@ -1276,7 +1276,13 @@ static void boundary_fluxes(orc_case* c, const double* u, double* sflux) {
         for (int v = 0; v < nv; ++v) uin[v] = U(u, e, bdof, v);
         const double sgn = INVJ(e, bdof) >= 0.0 ? 1.0 : -1.0;
         for (int comp = 0; comp < 2; ++comp) nrm[comp] = sgn * JA(e, bdof, dim, comp);
-        if (kind == ORC_BC_SLIP_WALL) {
+        const int bnode = idx * n + q;
+        const int mixed_dirichlet = kind == ORC_BC_MIXED_DIRICHLET_SLIP_WALL &&
+                                    c->bc_values[face][bnode * (nv + 1) + nv] != 0.0;
+        if (kind == ORC_BC_SLIP_WALL || kind == ORC_BC_NS_NOSLIP_WALL || kind == ORC_BC_NS_SLIP_WALL ||
+            (kind == ORC_BC_MIXED_DIRICHLET_SLIP_WALL && !mixed_dirichlet)) {
+          /* reflect_state, slip_wall_boundary.hpp:12-24; the NS walls use
+           * reflect_normal_velocity_2d (navier_stokes_wall_boundary.hpp:12-27) */
           const double rho = uin[0], vx = uin[1] / rho, vy = uin[2] / rho;
           const double nn = nrm[0] * nrm[0] + nrm[1] * nrm[1];
           const double vn = (vx * nrm[0] + vy * nrm[1]) / nn;
@@ -1293,8 +1299,9 @@ static void boundary_fluxes(orc_case* c, const double* u, double* sflux) {
             }
             uout[v] = acc / (double)n;
           }
+        } else if (kind == ORC_BC_MIXED_DIRICHLET_SLIP_WALL) {
+          for (int v = 0; v < nv; ++v) uout[v] = c->bc_values[face][bnode * (nv + 1) + v];
         } else {
-          const int bnode = idx * n + q;
           for (int v = 0; v < nv; ++v)
             uout[v] = c->bc_values[face] ? c->bc_values[face][bnode * nv + v] : c->bc_state[face][v];
         }
@@ -1428,6 +1435,98 @@ static void ensure_parabolic_scratch(orc_case* c) {
 
 #define G(arr, e, dof, var, dm) (arr)[(((e) * (int64_t)c->nn + (dof)) * nv + (var)) * d + (dm)]
 
+/* Boundary node q of element position idx along a face of a 2D mesh:
+ * element, boundary dof and the normal sign(J) Ja^dim there
+ * (utils/boundary_helper.hpp:69-150). */
+static void wall_node(const orc_case* c, int face, int idx, int q, int64_t* e, int* bdof,
+                      double nrm[2]) {
+  const int n = c->n, dim = face / 2, upper = face % 2;
+  const int ie = dim == 0 ? (upper ? c->nx - 1 : 0) : idx;
+  const int je = dim == 0 ? idx : (upper ? c->ny - 1 : 0);
+  *e = (int64_t)ie * c->ny + je;
+  const int bi = dim == 0 ? (upper ? n - 1 : 0) : q;
+  const int bj = dim == 0 ? q : (upper ? n - 1 : 0);
+  *bdof = bj * n + bi;
+  const double sgn = INVJ(*e, *bdof) >= 0.0 ? 1.0 : -1.0;
+  for (int comp = 0; comp < 2; ++comp) nrm[comp] = sgn * JA(*e, *bdof, dim, comp);
+}
+
+/* Gradient face values (apply_gradient_device hooks, dispatched by
+ * boundary_set.hpp:16-33):
+ *   adiabatic walls: the inner node's gradient variables with the velocity
+ *     set to zero (NSNoSlipAdiabaticWallBC, navier_stokes_wall_boundary.hpp:
+ *     81-114) or with its normal component removed (NSSlipAdiabaticWallBC,
+ *     :163-197 with remove_normal_gradient_velocity_2d :28-36);
+ *   Dirichlet: the gradient variables of the outer state
+ *     (DirichletBC::apply_gradient_device, dirichlet_boundary.hpp:122-162).
+ * Faces of the other kinds (slip wall, outflow, mixed) have no gradient hook
+ * and keep the zero the face array was cleared to (structured_solver.hpp:250). */
+static void wall_gradient_values(orc_case* c, const double* u) {
+  if (c->dim != 2) return;
+  const int n = c->n, nv = c->nvars, fn = face_nodes(c);
+  for (int face = 0; face < 4; ++face) {
+    const int kind = c->bc[face];
+    if (kind != ORC_BC_NS_NOSLIP_WALL && kind != ORC_BC_NS_SLIP_WALL && kind != ORC_BC_DIRICHLET)
+      continue;
+    const int count = face < 2 ? c->ny : c->nx;
+    for (int idx = 0; idx < count; ++idx)
+      for (int q = 0; q < n; ++q) {
+        int64_t e;
+        int bdof;
+        double nrm[2], s[4], qw[4];
+        wall_node(c, face, idx, q, &e, &bdof, nrm);
+        const int bnode = idx * n + q;
+        for (int v = 0; v < nv; ++v)
+          s[v] = kind != ORC_BC_DIRICHLET ? U(u, e, bdof, v)
+                 : c->bc_values[face]    ? c->bc_values[face][bnode * nv + v]
+                                         : c->bc_state[face][v];
+        gradient_variables(c, s, qw);
+        if (kind == ORC_BC_DIRICHLET) {
+          /* the outer state's variables as they are */
+        } else if (kind == ORC_BC_NS_NOSLIP_WALL) {
+          qw[1] = 0.0;
+          qw[2] = 0.0;
+        } else {
+          const double nn = nrm[0] * nrm[0] + nrm[1] * nrm[1];
+          const double vn = (qw[1] * nrm[0] + qw[2] * nrm[1]) / nn;
+          qw[1] -= vn * nrm[0];
+          qw[2] -= vn * nrm[1];
+        }
+        for (int v = 0; v < nv; ++v) SF(c->gflux, e, face * fn + q, v) = qw[v];
+      }
+  }
+}
+
+/* Viscous face values (apply_viscous_device hooks, boundary_set.hpp:35-51):
+ * the no-slip adiabatic wall takes the inner node's (-F_v).n for the
+ * momentum components and zero mass and energy flux
+ * (NSNoSlipAdiabaticWallBC, navier_stokes_wall_boundary.hpp:116-145);
+ * Dirichlet takes the inner node's (-F_v).n for every variable
+ * (dirichlet_boundary.hpp:164-194); the slip adiabatic wall's values are all
+ * zero (:199-216), as are those of kinds without a viscous hook. */
+static void wall_viscous_values(orc_case* c) {
+  if (c->dim != 2) return;
+  const int n = c->n, nv = c->nvars, d = c->dim, fn = face_nodes(c);
+  for (int face = 0; face < 4; ++face) {
+    const int kind = c->bc[face];
+    if (kind != ORC_BC_NS_NOSLIP_WALL && kind != ORC_BC_DIRICHLET) continue;
+    const int lo = kind == ORC_BC_DIRICHLET ? 0 : 1, hi = kind == ORC_BC_DIRICHLET ? nv : nv - 1;
+    const int count = face < 2 ? c->ny : c->nx;
+    for (int idx = 0; idx < count; ++idx)
+      for (int q = 0; q < n; ++q) {
+        int64_t e;
+        int bdof;
+        double nrm[2];
+        wall_node(c, face, idx, q, &e, &bdof, nrm);
+        for (int v = 0; v < nv; ++v)
+          SF(c->vsflux, e, face * fn + q, v) =
+              v >= lo && v < hi
+                  ? G(c->vflux, e, bdof, v, 0) * nrm[0] + G(c->vflux, e, bdof, v, 1) * nrm[1]
+                  : 0.0;
+      }
+  }
+}
+
 /* calc_gradients sequence, structured_solver.hpp:437-449 with
  * GradientVolumeIntegralFunctor (parabolic_gradient.hpp:40-79 / :502-527),
  * GradientPhysicalTransformFunctor (:121-150 / :567-584),
@@ -1510,6 +1609,7 @@ void orc_gradients(orc_case* c, const double* u, double* grad) {
       }
     }
   }
+  wall_gradient_values(c, u);
   /* surface: grad(face node) += (-+ lhat) * qhat * sign(J) Ja_dir */
   {
     const double f1 = c->lhat_l[0], f2 = c->lhat_r[n - 1];
@@ -1627,6 +1727,7 @@ static void viscous_terms(orc_case* c, const double* u, double* du) {
         }
       }
     }
+  wall_viscous_values(c);
   surface_integral_generic(c, c->vsflux, du, nv);
 }
 

@@ -95,6 +95,15 @@ void orc_gradients(orc_case* c, const double* u, double* gradient);
 #define ORC_BC_SLIP_WALL 1
 #define ORC_BC_OUTFLOW 2
 #define ORC_BC_DIRICHLET 3
+/* Navier-Stokes adiabatic walls (navier_stokes_wall_boundary.hpp:39-219):
+ * the inviscid flux of a slip wall, plus gradient and viscous face values */
+#define ORC_BC_NS_NOSLIP_WALL 4
+#define ORC_BC_NS_SLIP_WALL 5
+/* MixedDirichletSlipWallBC (dirichlet_boundary.hpp:198-275): values holds
+ * [face nodes][nvars + 1], the Dirichlet state and a flag, nonzero where the
+ * node takes the Dirichlet state and zero where it is a slip wall (the
+ * reference's use_dirichlet(coord, t)) */
+#define ORC_BC_MIXED_DIRICHLET_SLIP_WALL 6
 void orc_set_boundary(orc_case* c, int face, int kind, const double* state,
                       const double* values);
 

@@ -53,8 +53,9 @@ def test_invalid_configuration_is_rejected_with_a_message():
 
 @pytest.mark.parametrize("dim,faces,rc,word", [
     (2, [1, 0, 0, 0], -1, b"periodic on both faces"),   # -x wall, +x periodic
-    (2, [5, 5, 0, 0], -1, b"unknown boundary"),
-    (3, [1, 1, 0, 0, 0, 0], -2, b"2D compressible Euler"),
+    (2, [7, 7, 0, 0], -1, b"unknown boundary"),
+    (3, [1, 1, 0, 0, 0, 0], -2, b"2D meshes"),
+    (2, [4, 4, 0, 0], -1, b"Navier-Stokes boundary"),  # an adiabatic wall under Euler
 ])
 def test_boundary_configuration_errors_without_gpu(dim, faces, rc, word):
     """Face kinds are checked before any device work: a direction must be
@@ -71,12 +72,13 @@ def test_boundary_configuration_errors_without_gpu(dim, faces, rc, word):
     lib.dgsem_destroy(h)
 
 
-def test_navier_stokes_walls_not_built_without_gpu():
+def test_direction_kind_applies_to_both_faces_without_gpu():
+    """boundary[dim] sets both faces of a direction: a 3D mesh with x walls is
+    refused as not built, which shows both x faces took the wall kind."""
     lib = _abi.load()
     c = _abi.Config()
-    lib.dgsem_config_default(ctypes.byref(c), 2, 3)
-    c.equation = _abi.NAVIER_STOKES
-    c.boundary[0] = _abi.BC_SLIP_WALL  # both x faces through the per-direction kind
+    lib.dgsem_config_default(ctypes.byref(c), 3, 3)
+    c.boundary[0] = _abi.BC_SLIP_WALL
     h = ctypes.c_void_p()
     assert lib.dgsem_create(ctypes.byref(c), None, ctypes.byref(h)) == -2
     lib.dgsem_destroy(h)

@@ -154,3 +154,24 @@ def test_boundary_configurations_rejected(gpu):
     cfg = gs.cfg
     with pytest.raises(dg.DgsemError):
         dg.Batch([cfg, cfg])
+
+
+@pytest.mark.parametrize("volume", [dg.SPLIT_FORM, dg.SHOCK_CAPTURING])
+def test_mixed_dirichlet_slip_wall(gpu, volume):
+    """The double-Mach-reflection bottom wall: per node either an inflow
+    state or the slip-wall reflection (MixedDirichletSlipWallBC), here on the
+    lower y face, with a Dirichlet top and x inflow/outflow."""
+    print(f"case bc mixed volume={volume}", flush=True)
+    N, cells = 3, (8, 5)
+    n = N + 1
+    faces = [dg.BC_DIRICHLET, dg.BC_OUTFLOW, dg.BC_MIXED_DIRICHLET_SLIP_WALL, dg.BC_DIRICHLET]
+    oc, gs = make_bc_pair(N, cells, faces, volume=volume)
+    x_nodes = np.linspace(-1.0, 1.0, cells[0] * n)
+    vals = np.column_stack([np.tile(INFLOW, (cells[0] * n, 1)), (x_nodes < -0.4).astype(float)])
+    oc.set_boundary(2, O.BC_MIXED_DIRICHLET_SLIP_WALL, values=vals)
+    u = state(oc, seed=21)
+    with pytest.raises(dg.DgsemError):
+        gpu_rhs(gs, u)  # the mixed face's per-node states are required
+    gs.set_boundary_values(2, torch.from_numpy(vals).cuda())
+    ref = oc.rhs(u)
+    assert rel_l2(gpu_rhs(gs, u), ref) <= RHS_TOL
