@@ -89,7 +89,7 @@ def build(force: bool = True, verbose: bool = False, defines=(), out: str | None
     return out
 
 
-EXAMPLES = ["khi_shock_capturing"]
+EXAMPLES = ["khi_shock_capturing", "double_mach_reflection", "viscous_shock_tube"]
 
 
 def build_examples() -> None:

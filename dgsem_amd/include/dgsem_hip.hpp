@@ -20,7 +20,9 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/dgsem_amd.h"
@@ -140,15 +142,50 @@ class StructuredMesh {
   std::array<std::size_t, NDIM> n_cells_;
 };
 
+// Boundary conditions (src/boundary_condition/*.hpp).  Faces are ordered
+// -x, +x, -y, +y[, -z, +z] as in BoundarySet (boundary_set.hpp:175-240).
 struct PeriodicBC {
   static constexpr int kind = DGSEM_BC_PERIODIC;
+};
+struct SlipWallBC {  // slip_wall_boundary.hpp:10-66
+  static constexpr int kind = DGSEM_BC_SLIP_WALL;
+};
+struct OutflowBC {  // outflow_boundary.hpp:10-110
+  static constexpr int kind = DGSEM_BC_OUTFLOW;
+};
+struct NSNoSlipAdiabaticWallBC {  // navier_stokes_wall_boundary.hpp:39-146
+  static constexpr int kind = DGSEM_BC_NS_NOSLIP_WALL;
+};
+struct NSSlipAdiabaticWallBC {  // navier_stokes_wall_boundary.hpp:148-217
+  static constexpr int kind = DGSEM_BC_NS_SLIP_WALL;
+};
+// DirichletBC<Func> (dirichlet_boundary.hpp:11-196): func is either a state
+// indexable by variable (a constant outer state) or a callable
+// func(coord, t) -> state, evaluated here on the host at the boundary nodes
+// and uploaded whenever the solver is asked for a new time.
+template <class Func>
+struct DirichletBC {
+  static constexpr int kind = DGSEM_BC_DIRICHLET;
+  Func func;
+  explicit DirichletBC(Func func_) : func(func_) {}
+};
+// MixedDirichletSlipWallBC<Func, Predicate> (dirichlet_boundary.hpp:198-275):
+// func(coord, t) where use_dirichlet(coord, t), the slip wall elsewhere.
+template <class Func, class Predicate>
+struct MixedDirichletSlipWallBC {
+  static constexpr int kind = DGSEM_BC_MIXED_DIRICHLET_SLIP_WALL;
+  Func func;
+  Predicate use_dirichlet;
+  MixedDirichletSlipWallBC(Func func_, Predicate use_dirichlet_)
+      : func(func_), use_dirichlet(use_dirichlet_) {}
 };
 
 template <class... BCs>
 struct BoundarySet {
   static constexpr std::size_t NFACES = sizeof...(BCs);
   static constexpr std::array<int, sizeof...(BCs)> kinds{BCs::kind...};
-  explicit BoundarySet(BCs...) {}
+  std::tuple<BCs...> bcs;
+  explicit BoundarySet(BCs... b) : bcs(b...) {}
   BoundarySet() = default;
 };
 
@@ -211,9 +248,9 @@ class StructuredSolver {
                 "StructuredSolver requires one boundary condition per mesh face");
   using solution = Solution<Mesh, BasisT, Equations>;
 
-  StructuredSolver(const Equations& eq, const Mesh& mesh, const BoundarySetType& /*bcs*/,
+  StructuredSolver(const Equations& eq, const Mesh& mesh, const BoundarySetType& bcs,
                    hipStream_t stream = nullptr)
-      : eq_(eq), mesh_(mesh) {
+      : eq_(eq), mesh_(mesh), bcs_(bcs), stream_(stream) {
     dgsem_config c;
     dgsem_config_default(&c, (int)NDIMS, BasisT::polydeg);
     const auto nc = mesh.get_num_cells();
@@ -223,9 +260,9 @@ class StructuredSolver {
       c.cells[d] = (int)nc[d];
       c.domain_left[d] = L[d];
       c.domain_right[d] = R[d];
-      const int lo = BoundarySetType::kinds[2 * d], hi = BoundarySetType::kinds[2 * d + 1];
-      c.boundary[d] = (lo == hi) ? lo : -1;
     }
+    for (std::size_t f = 0; f < 2 * NDIMS; ++f) c.face_boundary[f] = BoundarySetType::kinds[f];
+    constant_dirichlet_states(c, std::make_index_sequence<2 * NDIMS>{});
     c.gamma = eq.get_gamma();
     if constexpr (equations::is_navier_stokes<Equations>::value) {
       c.equation = DGSEM_NAVIER_STOKES;
@@ -245,6 +282,18 @@ class StructuredSolver {
     check(dgsem_sizes(h, &E, &nn, &nv), h, "dgsem_sizes");
     n_elements_ = (std::size_t)E;
     n_dofs_ = (std::size_t)nn;
+    prepare_boundary_nodes(std::make_index_sequence<2 * NDIMS>{});
+    update_boundary_values(value_type(0));
+  }
+
+  // Re-evaluates the time-dependent boundary states (callable Dirichlet and
+  // mixed faces) at time t and uploads them; a no-op for constant data or an
+  // unchanged time.  Synchronises the solver's stream.
+  void update_boundary_values(value_type t) {
+    if (!has_timed_faces_ || (values_time_set_ && t == values_time_)) return;
+    evaluate_faces(t, std::make_index_sequence<2 * NDIMS>{});
+    values_time_ = t;
+    values_time_set_ = true;
   }
 
   // set_indicator_parameters, structured_solver.hpp:118-123
@@ -290,12 +339,14 @@ class StructuredSolver {
 
   // calc_rhs(sol, time), structured_solver.hpp:463-480
   void calc_rhs(solution& sol, value_type time) {
+    update_boundary_values(time);
     check(dgsem_rhs(h_.get(), sol.state(), sol.rhs(), time), h_.get(), "calc_rhs");
   }
 
-  // one fused SSPRK3 stage (RHS + parallel_ma2/ma3)
+  // one fused SSPRK3 stage (RHS at `time` + parallel_ma2/ma3)
   void ssprk3_stage(int stage, value_type dt, const value_type* u_base, const value_type* u_k,
-                    value_type* u_out) {
+                    value_type* u_out, value_type time = value_type(0)) {
+    update_boundary_values(time);
     check(dgsem_ssprk3_stage(h_.get(), stage, dt, u_base, u_k, u_out), h_.get(), "ssprk3_stage");
   }
   void synchronize() { check(dgsem_synchronize(h_.get()), h_.get(), "synchronize"); }
@@ -305,10 +356,113 @@ class StructuredSolver {
   struct Destroy {
     void operator()(dgsem_handle* h) const { dgsem_destroy(h); }
   };
+  struct FreeDevice {
+    void operator()(double* p) const { (void)hipFree(p); }
+  };
+
+  template <class BC>
+  static constexpr bool timed_dirichlet() {
+    if constexpr (BC::kind == DGSEM_BC_MIXED_DIRICHLET_SLIP_WALL) {
+      return true;
+    } else if constexpr (BC::kind == DGSEM_BC_DIRICHLET) {
+      return std::is_invocable_v<decltype(std::declval<BC>().func),
+                                 const std::array<value_type, NDIMS>&, value_type>;
+    } else {
+      return false;
+    }
+  }
+
+  template <std::size_t... F>
+  void constant_dirichlet_states(dgsem_config& c, std::index_sequence<F...>) {
+    (constant_state<F>(c), ...);
+  }
+  template <std::size_t F>
+  void constant_state(dgsem_config& c) {
+    using BC = std::tuple_element_t<F, decltype(bcs_.bcs)>;
+    if constexpr (BC::kind == DGSEM_BC_DIRICHLET && !timed_dirichlet<BC>()) {
+      const auto& s = std::get<F>(bcs_.bcs).func;
+      for (std::size_t v = 0; v < NVARS; ++v) c.face_state[F][v] = s[v];
+    }
+  }
+
+  // Boundary node coordinates of every face with time-dependent data, in
+  // the order the ABI reads per-node states: (element position along the
+  // face) * (N+1) + (node position along the face).
+  template <std::size_t... F>
+  void prepare_boundary_nodes(std::index_sequence<F...>) {
+    (prepare_face<F>(), ...);
+  }
+  template <std::size_t F>
+  void prepare_face() {
+    using BC = std::tuple_element_t<F, decltype(bcs_.bcs)>;
+    if constexpr (timed_dirichlet<BC>()) {
+      static_assert(NDIMS == 2, "non-periodic boundaries are built for 2D meshes");
+      has_timed_faces_ = true;
+      std::vector<double> x(n_elements_ * n_dofs_ * NDIMS);
+      check(dgsem_node_coordinates(h_.get(), x.data()), h_.get(), "node_coordinates");
+      const auto nc = mesh_.get_num_cells();
+      const std::size_t n = BasisT::NNodes, dir = F / 2, upper = F % 2;
+      const std::size_t count = dir == 0 ? nc[1] : nc[0];
+      auto& coords = face_coords_[F];
+      coords.clear();
+      for (std::size_t idx = 0; idx < count; ++idx)
+        for (std::size_t q = 0; q < n; ++q) {
+          const std::size_t ie = dir == 0 ? (upper ? nc[0] - 1 : 0) : idx;
+          const std::size_t je = dir == 0 ? idx : (upper ? nc[1] - 1 : 0);
+          const std::size_t i = dir == 0 ? (upper ? n - 1 : 0) : q;
+          const std::size_t j = dir == 0 ? q : (upper ? n - 1 : 0);
+          const std::size_t node = (ie * nc[1] + je) * n_dofs_ + j * n + i;
+          coords.push_back({x[node * NDIMS], x[node * NDIMS + 1]});
+        }
+      const std::size_t width = BC::kind == DGSEM_BC_MIXED_DIRICHLET_SLIP_WALL ? NVARS + 1 : NVARS;
+      face_host_[F].assign(coords.size() * width, 0.0);
+      void* p = nullptr;
+      if (hipMalloc(&p, face_host_[F].size() * sizeof(double)) != hipSuccess) throw std::bad_alloc();
+      face_dev_[F].reset(static_cast<double*>(p));
+      check(dgsem_set_boundary_values(h_.get(), (int)F, face_dev_[F].get()), h_.get(),
+            "set_boundary_values");
+    }
+  }
+
+  template <std::size_t... F>
+  void evaluate_faces(value_type t, std::index_sequence<F...>) {
+    (evaluate_face<F>(t), ...);
+  }
+  template <std::size_t F>
+  void evaluate_face(value_type t) {
+    using BC = std::tuple_element_t<F, decltype(bcs_.bcs)>;
+    if constexpr (timed_dirichlet<BC>()) {
+      const auto& bc = std::get<F>(bcs_.bcs);
+      const bool mixed = BC::kind == DGSEM_BC_MIXED_DIRICHLET_SLIP_WALL;
+      const std::size_t width = mixed ? NVARS + 1 : NVARS;
+      auto& host = face_host_[F];
+      // the previous upload may still be read by queued kernels
+      if (hipStreamSynchronize(stream_) != hipSuccess) throw std::runtime_error("hipStreamSynchronize");
+      for (std::size_t k = 0; k < face_coords_[F].size(); ++k) {
+        const auto& xc = face_coords_[F][k];
+        const auto s = bc.func(xc, t);
+        for (std::size_t v = 0; v < NVARS; ++v) host[k * width + v] = s[v];
+        if constexpr (BC::kind == DGSEM_BC_MIXED_DIRICHLET_SLIP_WALL)
+          host[k * width + NVARS] = bc.use_dirichlet(xc, t) ? 1.0 : 0.0;
+      }
+      if (hipMemcpyAsync(face_dev_[F].get(), host.data(), host.size() * sizeof(double),
+                         hipMemcpyHostToDevice, stream_) != hipSuccess ||
+          hipStreamSynchronize(stream_) != hipSuccess)
+        throw std::runtime_error("boundary state upload failed");
+    }
+  }
+
   Equations eq_;
   Mesh mesh_;
+  BoundarySetType bcs_;
+  hipStream_t stream_ = nullptr;
   std::unique_ptr<dgsem_handle, Destroy> h_;
   std::size_t n_elements_ = 0, n_dofs_ = 0;
+  bool has_timed_faces_ = false, values_time_set_ = false;
+  value_type values_time_ = value_type(0);
+  std::array<std::vector<std::array<double, NDIMS>>, 2 * NDIMS> face_coords_;
+  std::array<std::vector<double>, 2 * NDIMS> face_host_;
+  std::array<std::unique_ptr<double, FreeDevice>, 2 * NDIMS> face_dev_;
 };
 
 // --- time integration (SSPRK3.hpp:145-183, integrator_base.hpp:21-40) --------
@@ -318,11 +472,12 @@ class SSPRK3 {
   SSPRK3(const SolutionT& sol, const Mesh& mesh, T final_time = T(0))
       : tmp1_(sol.clone_shape()), tmp2_(sol.clone_shape()), mesh_(mesh), final_time_(final_time) {}
 
-  // u1 = u + dt L(u); u2 = 3/4 u + 1/4 u1 + dt/4 L(u1); u = 1/3 u + 2/3 u2 + 2dt/3 L(u2)
+  // u1 = u + dt L(u); u2 = 3/4 u + 1/4 u1 + dt/4 L(u1); u = 1/3 u + 2/3 u2 + 2dt/3 L(u2),
+  // with L evaluated at t, t + dt and t + dt/2 (SSPRK3.hpp:159-181)
   void step(Solver& solver, SolutionT& sol, T dt) {
-    solver.ssprk3_stage(1, dt, nullptr, sol.state(), tmp1_.state());
-    solver.ssprk3_stage(2, dt, sol.state(), tmp1_.state(), tmp2_.state());
-    solver.ssprk3_stage(3, dt, sol.state(), tmp2_.state(), sol.state());
+    solver.ssprk3_stage(1, dt, nullptr, sol.state(), tmp1_.state(), time_);
+    solver.ssprk3_stage(2, dt, sol.state(), tmp1_.state(), tmp2_.state(), time_ + dt);
+    solver.ssprk3_stage(3, dt, sol.state(), tmp2_.state(), sol.state(), time_ + T(0.5) * dt);
   }
   void solve(Solver& solver, SolutionT& sol, T dt) {
     while (time_ < final_time_) {
