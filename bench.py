@@ -157,6 +157,31 @@ def adjoint_measure(cells: int = 16, N: int = 3, steps: int = 200, every: int = 
             "ratio": (t2 - t1) / (t1 - t0), "objective": J}
 
 
+def vtu_measure(cells: int = 32, N: int = 3) -> dict:
+    """VTU output of a 3D TGV state (dgsem_write_vtu): wall time of one file
+    (GPU reorder + copy + write to a local temporary file) and its size."""
+    import tempfile
+
+    import torch
+
+    import dgsem_amd as dg
+
+    cfg = dg.SolverConfig(dim=3, polydeg=N, cells=(cells,) * 3, domain_left=(0, 0, 0),
+                          domain_right=(2 * math.pi,) * 3)
+    s = dg.StructuredSolver(cfg)
+    u = s.from_reference_layout(tgv_state(s.node_coordinates()))
+    torch.cuda.synchronize()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "tgv.vtu")
+        s.write_vtu(u, path)  # warm-up (allocations, page cache)
+        t0 = time.perf_counter()
+        s.write_vtu(u, path)
+        t = time.perf_counter() - t0
+        size = os.path.getsize(path)
+    return {"workload": f"tgv3d_{cells}^3_N{N}_vtu", "points": s.n_elements * s.ndofs,
+            "file_bytes": size, "ms": t * 1e3, "GB_per_s": size / t / 1e9}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -252,6 +277,7 @@ def main() -> int:
         out["batch"] = batch_measure()
         out["adjoint"] = adjoint_measure()
         out["adjoint_forward_step_ratio"] = out["adjoint"]["ratio"]
+        out["vtu"] = vtu_measure()
     if args.profile_copy_mib > 0:
         a = torch.ones(args.profile_copy_mib * (1 << 20) // 8, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)

@@ -33,6 +33,7 @@ SOURCES = [
     ("batch.hip", "hipcc", "batch", ()),
     ("adjoint.hip", "hipcc", "adjoint", ()),
     ("analysis.hip", "hipcc", "analysis", ()),
+    ("vtu.hip", "hipcc", "vtu", ()),
 ] + [("batch_inst.hip", "hipcc", f"batch_inst_b{b}", (f"DGSEM_BATCH_BLOCK={b}",))
      for b in (128, 256)] + [(f"{kind}_inst.hip", "hipcc", f"{kind}_inst_{d}d_n{n}",
        (f"DGSEM_INST_DIM={d}", f"DGSEM_INST_N={n}"))

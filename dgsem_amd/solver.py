@@ -266,6 +266,10 @@ class StructuredSolver:
     def synchronize(self) -> None:
         self._check(self._L.dgsem_synchronize(self._h))
 
+    def write_vtu(self, u, path: str) -> None:
+        """VTK XML file of Lagrange cells holding u (dgsem_write_vtu)."""
+        self._check(self._L.dgsem_write_vtu(self._h, _ptr(u), str(path).encode()))
+
 
 class SSPRK3:
     """Three-stage SSP Runge-Kutta in Shu-Osher form on a solver's state,
@@ -278,6 +282,11 @@ class SSPRK3:
         self.time = 0.0
         self.final_time = final_time
         self.iter = 0
+        self.observers = []
+        self.has_nan = False
+
+    def add_observer(self, observer) -> None:
+        self.observers.append(observer)
 
     def step(self, u, dt: float) -> None:
         s = self.solver
@@ -286,11 +295,18 @@ class SSPRK3:
         s.ssprk3_stage(3, dt, u, self.tmp2, u)
 
     def solve(self, u, dt: float) -> None:
-        """TimeIntegrator::solve (integrator_base.hpp:21-40): fixed dt while t < t_final."""
+        """TimeIntegrator::solve (integrator_base.hpp:21-40): fixed dt while
+        t < t_final; observers run before each step and may stop the run."""
         while self.time < self.final_time:
+            for obs in self.observers:
+                self.has_nan = bool(obs.on_step(self.iter, self.time)) or self.has_nan
             self.step(u, dt)
             self.time += dt
             self.iter += 1
+            if self.has_nan:
+                break
+        for obs in self.observers:
+            obs.on_finish(self.iter, self.time)
 
 
 class Batch:

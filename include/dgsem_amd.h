@@ -239,6 +239,14 @@ int dgsem_kinetic_energy_grad(dgsem_handle* h, const double* u, double scale, do
  * Reduced in a fixed order. */
 int dgsem_analyze(dgsem_handle* h, const double* u, double* out);
 
+/* Writes the device state u (internal layout) to a VTK XML unstructured-grid
+ * file of Lagrange cells, as VTUOutputObserver::write_vtu does
+ * (vtu_output_observer.hpp:62-214, vtu_output_observer_impl.cpp:15-117):
+ * cells x-fastest, points in VTK's Lagrange order (file version 2.2), point
+ * data rho, momenta, rhoE and pressure (gamma 1.4 as in the reference),
+ * appended raw binary.  Synchronises the handle's stream. */
+int dgsem_write_vtu(dgsem_handle* h, const double* u, const char* path);
+
 /* ---------------------------------------------------------------------
  * Batched ensembles.  The reference has no ensemble API: its drivers run
  * one StructuredSolver per process, one case after another.  A batch
