@@ -20,7 +20,8 @@ EXPORTED = (
     "dgsem_abi_version", "dgsem_config_default", "dgsem_create", "dgsem_destroy",
     "dgsem_last_error", "dgsem_sizes", "dgsem_set_indicator_parameters",
     "dgsem_node_coordinates", "dgsem_contravariant_vectors", "dgsem_inverse_jacobian",
-    "dgsem_set_boundary_values", "dgsem_write_vtu", "dgsem_pack_state", "dgsem_unpack_state", "dgsem_pack_state_layout",
+    "dgsem_set_boundary_values", "dgsem_write_vtu", "dgsem_reference_coordinates",
+    "dgsem_set_node_coordinates", "dgsem_pack_state", "dgsem_unpack_state", "dgsem_pack_state_layout",
     "dgsem_unpack_state_layout", "dgsem_rhs", "dgsem_get_alpha",
     "dgsem_ssprk3_stage", "dgsem_ssprk3_steps", "dgsem_synchronize",
     "dgsem_batch_create", "dgsem_batch_destroy", "dgsem_batch_last_error", "dgsem_batch_sizes",
@@ -75,6 +76,8 @@ def load(path: str | None = None):
     L.dgsem_pack_state.argtypes = [vp, vp, vp]
     L.dgsem_set_boundary_values.argtypes = [vp, C.c_int, vp]
     L.dgsem_write_vtu.argtypes = [vp, vp, C.c_char_p]
+    L.dgsem_reference_coordinates.argtypes = [vp, vp]
+    L.dgsem_set_node_coordinates.argtypes = [vp, vp]
     L.dgsem_unpack_state.argtypes = [vp, vp, vp]
     L.dgsem_pack_state_layout.argtypes = [vp, C.c_int, vp, vp]
     L.dgsem_unpack_state_layout.argtypes = [vp, C.c_int, vp, vp]

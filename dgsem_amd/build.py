@@ -34,6 +34,7 @@ SOURCES = [
     ("adjoint.hip", "hipcc", "adjoint", ()),
     ("analysis.hip", "hipcc", "analysis", ()),
     ("vtu.hip", "hipcc", "vtu", ()),
+    ("geometry.hip", "hipcc", "geometry", ()),
 ] + [("batch_inst.hip", "hipcc", f"batch_inst_b{b}", (f"DGSEM_BATCH_BLOCK={b}",))
      for b in (128, 256)] + [(f"{kind}_inst.hip", "hipcc", f"{kind}_inst_{d}d_n{n}",
        (f"DGSEM_INST_DIM={d}", f"DGSEM_INST_N={n}"))

@@ -296,6 +296,38 @@ class StructuredSolver {
     values_time_set_ = true;
   }
 
+  // A curvilinear mesh: the element geometry of `mapping` (any type with
+  // eval(std::array<T, NDIMS> reference_coordinate) -> physical coordinate,
+  // like LinearMapping, mapping.hpp:31-50) instead of the mesh's linear map,
+  // as StructuredElementInitializer<T, Basis, Mapping, NDIMS> builds it
+  // (data_container.hpp:729-768).  The mapping runs on the host; the metric
+  // terms are computed on the GPU.
+  template <class Mapping>
+    requires requires(const Mapping& m, std::array<value_type, NDIMS> x) { m.eval(x); }
+  StructuredSolver(const Equations& eq, const Mesh& mesh, const Mapping& mapping,
+                   const BoundarySetType& bcs, hipStream_t stream = nullptr)
+      : StructuredSolver(eq, mesh, bcs, stream) {
+    std::vector<double> x(n_elements_ * n_dofs_ * NDIMS);
+    check(dgsem_reference_coordinates(h_.get(), x.data()), h_.get(), "reference_coordinates");
+    for (std::size_t i = 0; i < n_elements_ * n_dofs_; ++i) {
+      std::array<value_type, NDIMS> r;
+      for (std::size_t d = 0; d < NDIMS; ++d) r[d] = x[i * NDIMS + d];
+      const auto p = mapping.eval(r);
+      for (std::size_t d = 0; d < NDIMS; ++d) x[i * NDIMS + d] = p[d];
+    }
+    void* dev = nullptr;
+    if (hipMalloc(&dev, x.size() * sizeof(double)) != hipSuccess) throw std::bad_alloc();
+    std::unique_ptr<double, FreeDevice> owned(static_cast<double*>(dev));
+    if (hipMemcpy(dev, x.data(), x.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      throw std::runtime_error("hipMemcpy to device failed");
+    check(dgsem_set_node_coordinates(h_.get(), owned.get()), h_.get(), "set_node_coordinates");
+    if (has_timed_faces_) {  // boundary nodes moved with the mapping
+      values_time_set_ = false;
+      prepare_boundary_nodes(std::make_index_sequence<2 * NDIMS>{});
+      update_boundary_values(value_type(0));
+    }
+  }
+
   // set_indicator_parameters, structured_solver.hpp:118-123
   void set_indicator_parameters(value_type amax, value_type amin, bool smooth) {
     check(dgsem_set_indicator_parameters(h_.get(), amax, amin, smooth ? 1 : 0), h_.get(),

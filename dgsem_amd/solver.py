@@ -155,6 +155,27 @@ class StructuredSolver:
         self._check(self._L.dgsem_node_coordinates(self._h, out.ctypes.data))
         return out
 
+    def reference_coordinates(self) -> np.ndarray:
+        """Reference-cube position of every node, [E][nn][dim] (the argument
+        of a mapping)."""
+        out = np.zeros((self.n_elements, self.ndofs, self.cfg.dim))
+        self._check(self._L.dgsem_reference_coordinates(self._h, out.ctypes.data))
+        return out
+
+    def set_node_coordinates(self, coords) -> None:
+        """Mapped node coordinates [E][nn][dim] (numpy or a GPU tensor); the
+        geometry is recomputed on the GPU."""
+        if not torch.is_tensor(coords):
+            coords = torch.from_numpy(np.ascontiguousarray(coords, dtype=np.float64)).cuda()
+        self._check(self._L.dgsem_set_node_coordinates(self._h, _ptr(coords)))
+
+    def apply_mapping(self, mapping) -> None:
+        """Curvilinear mesh from a vectorised mapping f(ref [..., dim]) ->
+        physical coordinates [..., dim], evaluated at the reference
+        positions of the nodes (StructuredElementInitializer with a user
+        Mapping)."""
+        self.set_node_coordinates(np.asarray(mapping(self.reference_coordinates()), dtype=np.float64))
+
     def contravariant_vectors(self) -> np.ndarray:
         out = np.zeros((self.n_elements, self.ndofs, self.cfg.dim, self.cfg.dim))
         self._check(self._L.dgsem_contravariant_vectors(self._h, out.ctypes.data))

@@ -133,6 +133,19 @@ int dgsem_set_indicator_parameters(dgsem_handle* h, double alpha_max, double alp
  * coordinates [E][nn][dim], contravariant vectors [E][nn][dim][dim],
  * inverse Jacobian [E][nn]. */
 int dgsem_node_coordinates(const dgsem_handle* h, double* host_out);
+
+/* Curvilinear meshes (the reference's StructuredElementInitializer with a
+ * user Mapping, data_container.hpp:729-768).  dgsem_reference_coordinates
+ * writes the reference-cube position of every node ([E][nn][dim], host),
+ * the argument calc_node_coordinates hands to Mapping::eval
+ * (data_container.hpp:193-221, :371-414).  dgsem_set_node_coordinates takes
+ * the mapped node coordinates ([E][nn][dim], device) and recomputes the
+ * Jacobian, the contravariant vectors (2D adjugate, 3D curl form), 1/J and
+ * the subcell normals on the GPU, bit-identical to the host construction
+ * (data_container.hpp:223-693, global_def.hpp:175-439); the host getters
+ * follow.  Nodal geometry mode only.  Synchronises the handle's stream. */
+int dgsem_reference_coordinates(const dgsem_handle* h, double* host_out);
+int dgsem_set_node_coordinates(dgsem_handle* h, const double* device_coords);
 int dgsem_contravariant_vectors(const dgsem_handle* h, double* host_out);
 int dgsem_inverse_jacobian(const dgsem_handle* h, double* host_out);
 

@@ -531,6 +531,8 @@ static inline int64_t eidx(const orc_case* c, int ie, int je, int ke) {
 /* Geometry: src/containers/data_container.hpp, src/base/global_def.hpp      */
 /* ------------------------------------------------------------------------ */
 
+static void build_metrics(orc_case* c);
+
 static void build_geometry(orc_case* c, const double* L, const double* R,
                            const int* periodic) {
   const int n = c->n, nn = c->nn, d = c->dim;
@@ -562,6 +564,28 @@ static void build_geometry(orc_case* c, const double* L, const double* R,
           }
         }
       }
+  build_metrics(c);
+  /* initialize_left_neighbor: :314-331 (2D), :695-724 (3D); -1 = none */
+  for (int ie = 0; ie < c->nx; ++ie)
+    for (int je = 0; je < c->ny; ++je)
+      for (int ke = 0; ke < (d == 3 ? c->nz : 1); ++ke) {
+        const int64_t e = eidx(c, ie, je, ke);
+        const int idx[3] = {ie, je, ke};
+        for (int dir = 0; dir < d; ++dir)
+          for (int comp = 0; comp < d; ++comp) {
+            int64_t v = idx[comp];
+            if (comp == dir)
+              v = idx[dir] > 0 ? idx[dir] - 1 : (periodic[dir] ? ncell[dir] - 1 : -1);
+            c->nbr[(e * d + dir) * d + comp] = v;
+          }
+      }
+}
+
+/* Jacobian matrix, contravariant vectors and inverse Jacobian from the node
+ * coordinates c->coords (whatever mapping produced them). */
+static void build_metrics(orc_case* c) {
+  const int n = c->n, nn = c->nn, d = c->dim;
+  const int64_t E = c->E;
 #define X(e, dof, comp) c->coords[((e) * (int64_t)nn + (dof)) * d + (comp)]
 #define JM(e, dof, dir, comp) c->jac[(((e) * (int64_t)nn + (dof)) * d + (dir)) * d + (comp)]
   /* calc_jacobian_matrix: :223-258 (2D), :416-482 (3D) */
@@ -656,20 +680,6 @@ static void build_geometry(orc_case* c, const double* L, const double* R,
     }
 #undef X
 #undef JM
-  /* initialize_left_neighbor: :314-331 (2D), :695-724 (3D); -1 = none */
-  for (int ie = 0; ie < c->nx; ++ie)
-    for (int je = 0; je < c->ny; ++je)
-      for (int ke = 0; ke < (d == 3 ? c->nz : 1); ++ke) {
-        const int64_t e = eidx(c, ie, je, ke);
-        const int idx[3] = {ie, je, ke};
-        for (int dir = 0; dir < d; ++dir)
-          for (int comp = 0; comp < d; ++comp) {
-            int64_t v = idx[comp];
-            if (comp == dir)
-              v = idx[dir] > 0 ? idx[dir] - 1 : (periodic[dir] ? ncell[dir] - 1 : -1);
-            c->nbr[(e * d + dir) * d + comp] = v;
-          }
-      }
 }
 
 /* SubcellNormalVectors<T,2/3>::initialize_host, global_def.hpp:175-251 (2D),
@@ -843,6 +853,15 @@ void orc_get_inverse_jacobian(const orc_case* c, double* o) {
 void orc_get_left_neighbors(const orc_case* c, int64_t* o) {
   memcpy(o, c->nbr, sizeof(int64_t) * (size_t)(c->E * c->dim * c->dim));
 }
+/* Node coordinates from any mapping (the reference's StructuredElementInitializer
+ * with a user Mapping, data_container.hpp:729-768): the metrics and subcell
+ * normals are recomputed from them as for the linear mapping. */
+void orc_set_coordinates(orc_case* c, const double* coords) {
+  memcpy(c->coords, coords, sizeof(double) * (size_t)(c->E * c->nn * c->dim));
+  build_metrics(c);
+  build_subcell_normals(c);
+}
+
 void orc_set_metrics(orc_case* c, const double* Ja, const double* invJ) {
   memcpy(c->Ja, Ja, sizeof(double) * (size_t)(c->E * c->nn * c->dim * c->dim));
   memcpy(c->invJ, invJ, sizeof(double) * (size_t)(c->E * c->nn));

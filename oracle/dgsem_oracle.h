@@ -64,6 +64,9 @@ void orc_get_inverse_jacobian(const orc_case* c, double* out); /* [E][nn] */
 void orc_get_left_neighbors(const orc_case* c, int64_t* out);  /* [E][dim][dim] */
 /* Overwrite the metric terms (for metric-sensitivity studies and for running
  * the restatement on exactly the metrics another implementation uses). */
+/* Replaces the node coordinates [E][nn][dim] (any mapping) and recomputes
+ * the Jacobian, contravariant vectors, 1/J and subcell normals. */
+void orc_set_coordinates(orc_case* c, const double* coords);
 void orc_set_metrics(orc_case* c, const double* contravariant,
                      const double* inverse_jacobian);
 

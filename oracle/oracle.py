@@ -61,6 +61,7 @@ def lib():
         getattr(L, name).argtypes = [vp, _dp]
     L.orc_get_left_neighbors.argtypes = [vp, _ip]
     L.orc_set_metrics.argtypes = [vp, _dp, _dp]
+    L.orc_set_coordinates.argtypes = [vp, _dp]
     L.orc_init_khi.argtypes = [vp, _dp]
     L.orc_init_tgv.argtypes = [vp, _dp]
     L.orc_init_sedov.argtypes = [vp, _dp, C.c_double, C.c_double, C.c_double]
@@ -195,6 +196,30 @@ class Case:
         out = np.zeros((self.E, self.dim, self.dim), dtype=np.int64)
         lib().orc_get_left_neighbors(self._h, out)
         return out
+
+    def reference_coordinates(self):
+        """Reference-cube position of every node before the mapping
+        (calc_node_coordinates, data_container.hpp:193-221, :371-414)."""
+        n = self.N + 1
+        nodes = basis(self.N)["nodes"]
+        cells = list(self.cells)
+        out = np.zeros((self.E, self.ndofs, self.dim))
+        dof = np.arange(self.ndofs)
+        nd = [dof % n, (dof // n) % n, dof // (n * n)]
+        e = np.arange(self.E)
+        if self.dim == 2:
+            idx = [e // cells[1], e % cells[1]]
+        else:
+            idx = [e // (cells[1] * cells[2]), (e // cells[2]) % cells[1], e % cells[2]]
+        for k in range(self.dim):
+            delta = 2.0 / cells[k]
+            off = -1.0 + idx[k].astype(np.float64) * delta + 0.5 * delta
+            out[..., k] = off[:, None] + 0.5 * delta * nodes[nd[k]][None, :]
+        return out
+
+    def set_coordinates(self, coords):
+        """Node coordinates of any mapping; metrics are recomputed."""
+        lib().orc_set_coordinates(self._h, np.ascontiguousarray(coords, dtype=np.float64))
 
     def set_metrics(self, Ja, invJ):
         lib().orc_set_metrics(self._h, np.ascontiguousarray(Ja, dtype=np.float64),
