@@ -194,6 +194,34 @@ constexpr bool pair_fence() {
   return N >= DGSEM_PAIR_FENCE_MIN_N;
 }
 
+// Lower and upper face neighbours of element e along direction dir on the
+// structured grid (element index (ie * ny + je) * nz + ke): the periodic
+// wrap, or -1 at a non-periodic boundary face -- what the neighbour table
+// holds, computed rather than loaded so that the neighbour traces do not
+// wait on a dependent load.
+template <int DIM>
+__device__ __forceinline__ void face_neighbours(const KParams& p, int64_t e, int dir, int64_t& lo,
+                                                int64_t& hi) {
+  const int ny = p.cells[1], nz = DIM == 3 ? p.cells[2] : 1;
+  const int ei = (int)e;
+  int idx, count, stride;
+  if (dir == 0) {
+    stride = ny * nz;
+    idx = ei / stride;
+    count = p.cells[0];
+  } else if (dir == 1) {
+    stride = nz;
+    idx = (ei / nz) % ny;
+    count = ny;
+  } else {
+    stride = 1;
+    idx = ei % nz;
+    count = nz;
+  }
+  lo = idx > 0 ? e - stride : (p.bc[2 * dir] ? -1 : e + (int64_t)(count - 1) * stride);
+  hi = idx < count - 1 ? e + stride : (p.bc[2 * dir + 1] ? -1 : e - (int64_t)(count - 1) * stride);
+}
+
 // Outer state of a non-periodic face of a 2D compressible-Euler mesh, seen
 // from the line of the element that owns the face (face = 2 * dir + upper):
 //   slip wall  slip_wall_boundary.hpp:12-62: the inner state with the normal
