@@ -195,31 +195,28 @@ constexpr bool pair_fence() {
 }
 
 // Lower and upper face neighbours of element e along direction dir on the
-// structured grid (element index (ie * ny + je) * nz + ke): the periodic
-// wrap, or -1 at a non-periodic boundary face -- what the neighbour table
-// holds, computed rather than loaded so that the neighbour traces do not
-// wait on a dependent load.
-template <int DIM>
-__device__ __forceinline__ void face_neighbours(const KParams& p, int64_t e, int dir, int64_t& lo,
-                                                int64_t& hi) {
-  const int ny = p.cells[1], nz = DIM == 3 ? p.cells[2] : 1;
+// structured grid of cells[] (element index (ie * ny + je) * nz + ke): the
+// periodic wrap, or -1 at a non-periodic boundary face (BCS: kinds from
+// p.bc) -- what the neighbour table holds, computed rather than loaded so
+// that the neighbour traces do not wait on a dependent load.  The grid
+// sizes come in as values: selecting them per thread from the parameter
+// block would index it dynamically and turn the reads into vector loads.
+template <int DIM, bool BCS>
+__device__ __forceinline__ void face_neighbours(const KParams& p, int nx, int ny, int nz, int64_t e,
+                                                int dir, int64_t& lo, int64_t& hi) {
   const int ei = (int)e;
-  int idx, count, stride;
-  if (dir == 0) {
-    stride = ny * nz;
-    idx = ei / stride;
-    count = p.cells[0];
-  } else if (dir == 1) {
-    stride = nz;
-    idx = (ei / nz) % ny;
-    count = ny;
-  } else {
-    stride = 1;
-    idx = ei % nz;
-    count = nz;
+  const int stride = dir == 0 ? ny * nz : (dir == 1 ? nz : 1);
+  const int count = dir == 0 ? nx : (dir == 1 ? ny : nz);
+  const int idx = dir == 0 ? ei / stride : (dir == 1 ? (ei / nz) % ny : ei % nz);
+  bool wrap_lo = true, wrap_hi = true;
+  if constexpr (BCS) {
+    const int blo = dir == 0 ? p.bc[0] : (dir == 1 ? p.bc[2] : p.bc[4]);
+    const int bhi = dir == 0 ? p.bc[1] : (dir == 1 ? p.bc[3] : p.bc[5]);
+    wrap_lo = blo == kBcPeriodic;
+    wrap_hi = bhi == kBcPeriodic;
   }
-  lo = idx > 0 ? e - stride : (p.bc[2 * dir] ? -1 : e + (int64_t)(count - 1) * stride);
-  hi = idx < count - 1 ? e + stride : (p.bc[2 * dir + 1] ? -1 : e - (int64_t)(count - 1) * stride);
+  lo = idx > 0 ? e - stride : (wrap_lo ? e + (int64_t)(count - 1) * stride : -1);
+  hi = idx < count - 1 ? e + stride : (wrap_hi ? e - (int64_t)(count - 1) * stride : -1);
 }
 
 // Outer state of a non-periodic face of a 2D compressible-Euler mesh, seen
