@@ -112,3 +112,42 @@ def test_viscous_terms_are_conservative():
     totals = np.einsum("ed,edv->v", vol, visc)
     scale = np.einsum("ed,edv->v", vol, np.abs(visc)).max()
     assert np.max(np.abs(totals)) <= 1e-13 * scale
+
+
+@pytest.mark.parametrize("dim,N,cells", [(2, 3, (4, 4)), (3, 2, (3, 3, 3))])
+def test_viscous_interface_flux_is_the_upper_elements(dim, N, cells):
+    """The reference's viscous interface flux (parabolic_viscous.hpp:319-333,
+    :571-585) is one-sided: the upper element's (-F_v).n at the face, written
+    to both sides.  A change of one element's interior nodes leaves its face
+    traces, and with them every neighbour's BR1 gradient and Euler face flux,
+    unchanged; it moves the element's own gradient and viscous flux at its
+    faces.  So the elements below it (for which it is the upper element) see
+    the change, the elements above it do not, and nothing else does.  A
+    two-sided average would move both neighbours; the lower element's value
+    would move only the ones above."""
+    c = O.Case(dim, N, cells, [0.0] * dim, [TWO_PI] * dim, eq=O.NAVIER_STOKES, mu=0.05,
+               prandtl=0.72)
+    x = c.coordinates()
+    rho = 1.0 + 0.1 * np.sin(x[..., 0]) * np.cos(x[..., 1])
+    v = np.stack([0.2 * np.cos(x[..., (k + 1) % dim]) for k in range(dim)], axis=-1)
+    p = 1.0 + 0.05 * np.sin(x[..., 1])
+    u = state_from_prims(rho, v, p)
+    n = N + 1
+    idx = np.indices((n,) * dim).reshape(dim, -1)[::-1]  # (i, j[, k]) of dof, i fastest
+    interior = np.all((idx > 0) & (idx < N), axis=0)
+    grid = np.array(cells)
+    target = grid // 2
+    flat = lambda ijk: int(ijk[0] * cells[1] + ijk[1]) if dim == 2 else \
+        int((ijk[0] * cells[1] + ijk[1]) * cells[2] + ijk[2])
+    a = flat(target)
+    up = u.copy()
+    rng = np.random.default_rng(7)
+    up[a, interior, 1:dim + 1] += 1e-3 * rng.standard_normal((interior.sum(), dim))
+    delta = np.abs(c.rhs(up) - c.rhs(u)).reshape(c.E, -1).max(axis=1)
+    below = {flat(target - np.eye(dim, dtype=int)[d]) for d in range(dim)}
+    above = {flat(target + np.eye(dim, dtype=int)[d]) for d in range(dim)}
+    assert delta[a] > 1e-6
+    for e in below:
+        assert delta[e] > 1e-7, ("lower neighbour unchanged", e)
+    for e in set(range(c.E)) - below - {a}:
+        assert delta[e] == 0.0, ("element changed", e, e in above)
