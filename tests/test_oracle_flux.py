@@ -116,3 +116,59 @@ def test_hllc_consistency_rotation_upwinding(dim):
     np.testing.assert_allclose(O.flux_hllc(dim, G, a, b, n), euler_flux_n(dim, a, n), rtol=1e-13)
     # with the normal reversed the same flow is upwinded from the other side
     np.testing.assert_allclose(O.flux_hllc(dim, G, b, a, -n), euler_flux_n(dim, a, -n), rtol=1e-13)
+
+
+def normal_shock(dim, rng, mach, sign):
+    """An isolated shock of the family moving along +n (sign 1) or -n (sign -1)
+    relative to the fluid ahead of it, from the normal-shock relations, with
+    a tangential velocity that the shock does not change.  Returns (u_l, u_r,
+    n, shock speed along n)."""
+    n = rng.standard_normal(dim)
+    n /= np.linalg.norm(n)
+    t = rng.standard_normal(dim)
+    t -= (t @ n) * n
+    rho_a, p_a, vn_a = 0.8 + 0.4 * rng.random(), 0.7 + 0.6 * rng.random(), rng.standard_normal()
+    c_a = math.sqrt(G * p_a / rho_a)
+    ratio = (G + 1) * mach ** 2 / ((G - 1) * mach ** 2 + 2)
+    rho_b = rho_a * ratio
+    p_b = p_a * (1 + 2 * G / (G + 1) * (mach ** 2 - 1))
+    S = vn_a + sign * mach * c_a
+    vn_b = S - (S - vn_a) / ratio
+    ahead = cons(dim, rho_a, vn_a * n + t, p_a)
+    behind = cons(dim, rho_b, vn_b * n + t, p_b)
+    # the shock moves into the state ahead of it
+    return (behind, ahead, n, S) if sign > 0 else (ahead, behind, n, S)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_hllc_resolves_isolated_shocks_and_contacts(dim):
+    """With the reference's wave-speed bounds (min/max of the one-sided speeds
+    and the Roe-averaged ones, numerical_flux.hpp:331-335) the Roe speed of an
+    isolated shock is its Rankine-Hugoniot speed, and the HLLC middle state
+    then collapses onto the exact Riemann solution: the flux through the
+    interface is the physical flux of the state on the shock's side of
+    x/t = 0.  Contacts (equal pressure and normal velocity) are resolved
+    exactly too.  Plain one-sided bounds (v -/+ c) or a wrong star state break
+    the shock case."""
+    rng = np.random.default_rng(5)
+    for _ in range(12):
+        for mach in (1.3, 2.5, 7.0):
+            for sign in (1, -1):
+                ul, ur, n, S = normal_shock(dim, rng, mach, sign)
+                upwind = ul if S > 0 else ur
+                f = O.flux_hllc(dim, G, ul, ur, n)
+                np.testing.assert_allclose(f, euler_flux_n(dim, upwind, n), rtol=1e-11, atol=1e-11)
+                # the interface normal's length scales the flux
+                np.testing.assert_allclose(O.flux_hllc(dim, G, ul, ur, 2.5 * n), 2.5 * f,
+                                           rtol=1e-12, atol=1e-12)
+        n = rng.standard_normal(dim)
+        n /= np.linalg.norm(n)
+        vn, p = rng.standard_normal(), 0.5 + rng.random()
+        tl, tr = rng.standard_normal(dim), rng.standard_normal(dim)
+        tl -= (tl @ n) * n
+        tr -= (tr @ n) * n
+        ul = cons(dim, 0.5 + rng.random(), vn * n + tl, p)
+        ur = cons(dim, 1.5 + rng.random(), vn * n + tr, p)
+        upwind = ul if vn > 0 else ur
+        np.testing.assert_allclose(O.flux_hllc(dim, G, ul, ur, n), euler_flux_n(dim, upwind, n),
+                                   rtol=1e-12, atol=1e-12)
