@@ -1,0 +1,436 @@
+// Node-parallel variant of the fused RHS / SSPRK3 stage kernel for 3D
+// split-form elements of high polynomial degree (periodic meshes, no shock
+// capturing).
+//
+// The line-thread kernel (rhs_kernel) keeps all N+1 node sums of a line in
+// registers; at N = 7 that is 40 accumulators plus the line's metric rows,
+// which takes the kernel past 400 registers and down to one wave per SIMD.
+// Here one workgroup owns one element and one thread owns one node: it
+// walks the three lines through its node and sums D_split(x, m) f(u_x, u_m)
+// over the partners m, evaluating every pair flux itself.  Each pair is thus
+// evaluated twice, once from either end (the flux is symmetric in its two
+// nodes to the bit, so both ends agree and conservation is exact), but a
+// thread holds only its node's five sums and one partner, so the kernel
+// fits four waves per SIMD with two elements per CU, and there is no
+// cross-thread accumulation: the node's result goes straight from registers
+// to memory.
+//
+// At N = 7 the split-form sums cancel terms some thousand times larger than
+// their result, so the RHS carries the last bits of every term and the
+// order of the sums at the 1e-12 level.  This kernel therefore takes each
+// node's terms in the reference's order and rounds every operation of the
+// pair flux, the face flux and the sums as its scalar code does (true
+// quotients, no fused multiply-adds).  Against a scalar restatement of the
+// reference's algorithm this kernel stays within a few 1e-13 on the hardest
+// small case (TGV, 3x2x4 elements), where the line kernel's faster
+// arithmetic is at 2e-12.
+//
+// Reference kernels replaced: the same as rhs_kernel's split-form path
+// (volume_flux.hpp:221-410, surface_flux.hpp:45-271,
+// surface_integral.hpp:35-154, jacobian.hpp:22-105, SSPRK3.hpp:12-143).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "physics.hpp"
+
+namespace dgsem {
+namespace detail {
+
+#ifndef DGSEM_NODE_KERNEL_MIN_N
+#define DGSEM_NODE_KERNEL_MIN_N 7
+#endif
+
+// 3D split-form elements of degree N run the node-parallel kernel.
+template <int DIM, int N, bool SC, bool BCS>
+constexpr bool use_node_kernel() {
+  return DIM == 3 && !SC && !BCS && N >= DGSEM_NODE_KERNEL_MIN_N;
+}
+
+// The functions below round every operation as the reference's scalar code
+// does (numerical_flux.hpp:520-594, compressible_euler3D.hpp:26-166,
+// math_utils.hpp:8-24): true quotients and no fused multiply-adds.  At high
+// degree the split-form sums cancel terms some thousand times larger than
+// their result, so the last bits of each pair flux and the order of the sums
+// show in the RHS at the 1e-12 level; with the reference's roundings and its
+// summation order the kernel reproduces its results.
+
+// rho, v, beta = rho / (2 p), |v|^2 / 2, p and c of a conservative state.
+__device__ __forceinline__ NodePrims<3> node_prims_exact(double g, const double* u) {
+#pragma clang fp contract(off)
+  NodePrims<3> q;
+  q.rho = u[0];
+  q.v[0] = u[1] / u[0];
+  q.v[1] = u[2] / u[0];
+  q.v[2] = u[3] / u[0];
+  const double v2 = q.v[0] * q.v[0] + q.v[1] * q.v[1] + q.v[2] * q.v[2];
+  q.kin = 0.5 * v2;
+  q.p = (g - 1.0) * (u[4] - 0.5 * u[0] * v2);
+  q.beta = 0.5 * u[0] / q.p;
+  q.c = sqrt(g * q.p / u[0]);
+  return q;
+}
+
+// Chandrashekar flux, symmetric in (a, b) to the bit: the series test uses
+// ((x - y) / (x + y))^2 (whose rounding cannot reach the polynomial, which
+// is 2 to within 1e-4), and the log branch orders its arguments by value.
+__device__ __forceinline__ void flux_chandrashekar_exact(double inv_gm1, const NodePrims<3>& a,
+                                                         const NodePrims<3>& b, const double* n,
+                                                         double* f) {
+#pragma clang fp contract(off)
+  const double vna = a.v[0] * n[0] + a.v[1] * n[1] + a.v[2] * n[2];
+  const double vnb = b.v[0] * n[0] + b.v[1] * n[1] + b.v[2] * n[2];
+  const double sr = a.rho + b.rho, sb = a.beta + b.beta;
+  const double xr = a.rho - b.rho, xb = a.beta - b.beta;
+  const double r1 = recip((sr * sr) * (sb * sb));
+  const double f2r = (xr * xr) * ((sb * sb) * r1), f2b = (xb * xb) * ((sr * sr) * r1);
+  double rho_mean, beta_mean;
+  if (f2r < 1.0e-4) {
+    rho_mean = sr / (2.0 + f2r * (2.0 / 3.0 + f2r * (2.0 / 5.0 + f2r * (2.0 / 7.0))));
+  } else {
+    const double lo = dmin(a.rho, b.rho), hi = dmax(a.rho, b.rho);
+    rho_mean = (hi - lo) / log(hi / lo);
+  }
+  if (f2b < 1.0e-4) {
+    beta_mean = sb / (2.0 + f2b * (2.0 / 3.0 + f2b * (2.0 / 5.0 + f2b * (2.0 / 7.0))));
+  } else {
+    const double lo = dmin(a.beta, b.beta), hi = dmax(a.beta, b.beta);
+    beta_mean = (hi - lo) / log(hi / lo);
+  }
+  const double p_mean = 0.5 * sr / sb;
+  const double vsq = a.kin + b.kin;
+  const double f1 = rho_mean * 0.5 * (vna + vnb);
+  double fe = f1 * 0.5 * (inv_gm1 / beta_mean - vsq);
+  f[0] = f1;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double vavg = 0.5 * (a.v[k] + b.v[k]);
+    f[k + 1] = f1 * vavg + p_mean * n[k];
+    fe = fe + f[k + 1] * vavg;
+  }
+  f[4] = fe;
+}
+
+// n . (F_x, F_y, F_z)(u) (compressible_euler3D.hpp:26-67) and the state's
+// pressure.
+__device__ __forceinline__ double physical_flux_exact(double g, const double* u, const double* n,
+                                                      double* f) {
+#pragma clang fp contract(off)
+  const double v0 = u[1] / u[0], v1 = u[2] / u[0], v2 = u[3] / u[0];
+  const double p = (g - 1.0) * (u[4] - 0.5 * u[0] * (v0 * v0 + v1 * v1 + v2 * v2));
+  const double fx[5] = {u[1], u[1] * v0 + p, u[1] * v1, u[1] * v2, v0 * (u[4] + p)};
+  const double fy[5] = {u[2], u[2] * v0, u[2] * v1 + p, u[2] * v2, v1 * (u[4] + p)};
+  const double fz[5] = {u[3], u[3] * v0, u[3] * v1, u[3] * v2 + p, v2 * (u[4] + p)};
+#pragma unroll
+  for (int v = 0; v < 5; ++v) f[v] = n[0] * fx[v] + n[1] * fy[v] + n[2] * fz[v];
+  return p;
+}
+
+// Local Lax-Friedrichs flux from conservative states (numerical_flux.hpp:54-68
+// with compressible_euler3D.hpp:121-166).
+__device__ __forceinline__ void flux_llf_exact(double g, const double* ul, const double* ur,
+                                               const double* n, double* f) {
+#pragma clang fp contract(off)
+  double fl[5], fr[5], vn[2], a[2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const double* u = s == 0 ? ul : ur;
+    const double p = physical_flux_exact(g, u, n, s == 0 ? fl : fr);
+    a[s] = sqrt(g * p / u[0]);
+    vn[s] = (u[1] / u[0]) * n[0] + (u[2] / u[0]) * n[1] + (u[3] / u[0]) * n[2];
+  }
+  const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  const double lam = dmax(fabs(vn[0]), fabs(vn[1])) + dmax(a[0], a[1]) * nn;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) f[v] = 0.5 * (fl[v] + fr[v]) - 0.5 * lam * (ur[v] - ul[v]);
+}
+
+// HLLC flux with a normal vector (a build extension: the reference has
+// HLLC for 1D Euler only, numerical_flux.hpp:283-378, whose structure this
+// follows in the frame of the unit normal, scaled by |n|).
+__device__ __forceinline__ void flux_hllc_exact(double g, const double* ul, const double* ur,
+                                                const double* n, double* f) {
+#pragma clang fp contract(off)
+  double nh[3], vl[3], vr[3], fl[5], fr[5];
+  const double nrm = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) nh[d] = n[d] / nrm;
+  const double rho_ll = ul[0], rho_rr = ur[0], rhoE_ll = ul[4], rhoE_rr = ur[4];
+  double v2l = 0.0, v2r = 0.0, vn_l = 0.0, vn_r = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    vl[d] = ul[d + 1] / rho_ll;
+    vr[d] = ur[d + 1] / rho_rr;
+    v2l += vl[d] * vl[d];
+    v2r += vr[d] * vr[d];
+    vn_l += vl[d] * nh[d];
+    vn_r += vr[d] * nh[d];
+  }
+  const double E_ll = rhoE_ll / rho_ll, E_rr = rhoE_rr / rho_rr;
+  const double p_ll = (g - 1.0) * (rhoE_ll - 0.5 * rho_ll * v2l);
+  const double p_rr = (g - 1.0) * (rhoE_rr - 0.5 * rho_rr * v2r);
+  const double c_ll = sqrt(g * p_ll / rho_ll), c_rr = sqrt(g * p_rr / rho_rr);
+  physical_flux_exact(g, ul, nh, fl);
+  physical_flux_exact(g, ur, nh, fr);
+  const double srl = sqrt(rho_ll), srr = sqrt(rho_rr), ssum = srl + srr;
+  double vroe2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double vroe_d = (srl * vl[d] + srr * vr[d]) / ssum;
+    vroe2 += vroe_d * vroe_d;
+  }
+  const double vel_roe = (srl * vn_l + srr * vn_r) / ssum;
+  const double ekin_roe = 0.5 * vroe2;
+  const double H_ll = (rhoE_ll + p_ll) / rho_ll, H_rr = (rhoE_rr + p_rr) / rho_rr;
+  const double H_roe = (srl * H_ll + srr * H_rr) / ssum;
+  const double c_roe = sqrt((g - 1.0) * (H_roe - ekin_roe));
+  const double Ssl = dmin(vn_l - c_ll, vel_roe - c_roe);
+  const double Ssr = dmax(vn_r + c_rr, vel_roe + c_roe);
+  const double sMu_L = Ssl - vn_l, sMu_R = Ssr - vn_r;
+  if (Ssl >= 0) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) f[i] = fl[i];
+  } else if (Ssr <= 0) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) f[i] = fr[i];
+  } else {
+    const double SStar = (p_rr - p_ll + rho_ll * vn_l * sMu_L - rho_rr * vn_r * sMu_R) /
+                         (rho_ll * sMu_L - rho_rr * sMu_R);
+    const bool left = SStar >= 0;
+    const double* uk = left ? ul : ur;
+    const double* fk = left ? fl : fr;
+    const double* vk = left ? vl : vr;
+    const double rk = left ? rho_ll : rho_rr, vnk = left ? vn_l : vn_r;
+    const double Ek = left ? E_ll : E_rr, pk = left ? p_ll : p_rr;
+    const double Sk = left ? Ssl : Ssr, sMu = left ? sMu_L : sMu_R;
+    const double densStar = rk * sMu / (Sk - SStar);
+    const double enerStar = Ek + (SStar - vnk) * (SStar + pk / (rk * sMu));
+    f[0] = fk[0] + Sk * (densStar - uk[0]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double mstar = densStar * (vk[d] + (SStar - vnk) * nh[d]);
+      f[d + 1] = fk[d + 1] + Sk * (mstar - uk[d + 1]);
+    }
+    f[4] = fk[4] + Sk * (densStar * enerStar - uk[4]);
+  }
+#pragma unroll
+  for (int i = 0; i < 5; ++i) f[i] *= nrm;
+}
+
+template <int N, bool NODAL, int SF>
+__global__ void __launch_bounds__((N + 1) * (N + 1) * (N + 1)) __attribute__((amdgpu_waves_per_eu(4)))
+    rhs_node_kernel(const KParams p) {
+  constexpr int DIM = 3, n = N + 1, NV = 5, NN = n * n * n;
+  constexpr int PN = NN + NN / n;  // padded plane (lds_pos)
+  constexpr int DS = n + 1;        // padded row of the D_split table
+  __shared__ double s_q[6 * PN];                 // rho, v[3], beta, kin
+  __shared__ double s_ja[NODAL ? 9 * PN : 1];    // Ja^d_c / 2
+  __shared__ double s_D[n * DS];
+
+  const int a = threadIdx.x;
+  const int64_t e = xcd_block(blockIdx.x, gridDim.x);
+  const int idx[3] = {a % n, (a / n) % n, a / (n * n)};
+  constexpr int stride[3] = {1, n, n * n};
+  const int pa = lds_pos<N>(a);
+
+  // 1. loads: the node's state, metric rows and 1/J; per direction, the face
+  // neighbour's trace facing this node (lower face: the lower neighbour's
+  // node with index N; upper face: the upper neighbour's node with index 0,
+  // and its metric row and 1/J, whose sign and row make the interface
+  // normal).  Interior nodes read their own node instead, so every load is
+  // unconditional and hits lines the block reads anyway.
+  double u[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) u[v] = p.u[(e * NV + v) * NN + a];
+  double jm[3][3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      jm[d][c] = NODAL ? p.Ja[((e * 3 + d) * 3 + c) * NN + a] : p.Ja[(e * 3 + d) * 3 + c];
+  const double ij = NODAL ? p.invJ[e * NN + a] : p.invJ[e];
+  double nb_u[3][NV], nb_ja[3][3], nb_ij[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int64_t lo, hi;
+    face_neighbours<DIM, false>(p, p.cells[0], p.cells[1], p.cells[2], e, d, lo, hi);
+    const bool lower = idx[d] == 0, upper = idx[d] == N;
+    const int64_t ne = lower ? lo : (upper ? hi : e);
+    const int nn = lower ? a + N * stride[d] : (upper ? a - N * stride[d] : a);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) nb_u[d][v] = p.u[(ne * NV + v) * NN + nn];
+    const int64_t je = upper ? hi : e;
+    const int jn = upper ? nn : a;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      nb_ja[d][c] = NODAL ? p.Ja[((je * 3 + d) * 3 + c) * NN + jn] : p.Ja[(je * 3 + d) * 3 + c];
+    nb_ij[d] = NODAL ? p.invJ[je * NN + jn] : p.invJ[je];
+  }
+
+  // 2. the node's primitives and halved metric rows into LDS
+  const NodePrims<DIM> q = node_prims_exact(p.gamma, u);
+  s_q[pa] = q.rho;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) s_q[(1 + c) * PN + pa] = q.v[c];
+  s_q[4 * PN + pa] = q.beta;
+  s_q[5 * PN + pa] = q.kin;
+  if constexpr (NODAL) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) s_ja[(d * 3 + c) * PN + pa] = 0.5 * jm[d][c];
+  }
+  if (a < n * n) s_D[(a / n) * DS + a % n] = p.Ds[a / n][a % n];
+  __syncthreads();
+
+  // 3. numerical fluxes on the faces this node lies on, lifted after the
+  // volume terms below
+  const double own_sg = ij >= 0.0 ? 1.0 : -1.0;
+  double fs[3][NV];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) fs[d][v] = 0.0;
+    const bool lower = idx[d] == 0, upper = idx[d] == N;
+    if (lower || upper) {
+      const double nb_sg = nb_ij[d] >= 0.0 ? 1.0 : -1.0;
+      double ul[NV], ur[NV], nrm[3];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        ul[v] = lower ? nb_u[d][v] : u[v];
+        ur[v] = lower ? u[v] : nb_u[d][v];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) nrm[c] = lower ? own_sg * jm[d][c] : nb_sg * nb_ja[d][c];
+      if constexpr (SF == kFluxLLF) {
+        flux_llf_exact(p.gamma, ul, ur, nrm, fs[d]);
+      } else {
+        flux_hllc_exact(p.gamma, ul, ur, nrm, fs[d]);
+      }
+    }
+  }
+
+  // 4. flux differencing.  The reference (volume_flux.hpp:296-410) visits
+  // the nodes in memory order and adds each pair's two terms as it goes, so
+  // a node's sum takes its terms in the order: partners below it on its
+  // z-line, y-line and x-line, then partners above it on its x-, y- and
+  // z-lines, each line in increasing position.  Every node has 3N terms, so
+  // each lane walks its own sequence in one uniform loop, and the face terms
+  // follow in direction order (surface_integral.hpp:82-154).
+  const double* const DsT = s_D;
+  const int ix = idx[0], iy = idx[1], iz = idx[2];
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+#pragma unroll 1
+  for (int t = 0; t < 3 * N; ++t) {
+    int d, m;
+    if (t < iz) {
+      d = 2; m = t;
+    } else if (t < iz + iy) {
+      d = 1; m = t - iz;
+    } else if (t < iz + iy + ix) {
+      d = 0; m = t - iz - iy;
+    } else if (t < iz + iy + N) {
+      d = 0; m = t - iz - iy + 1;
+    } else if (t < iz + 2 * N) {
+      d = 1; m = t - iz - N + 1;
+    } else {
+      d = 2; m = t - 2 * N + 1;
+    }
+    const int xd = d == 0 ? ix : (d == 1 ? iy : iz);
+    const int sd = d == 0 ? 1 : (d == 1 ? n : n * n);
+    const int pb = lds_pos<N>(a + (m - xd) * sd);
+    NodePrims<DIM> qb;
+    qb.rho = s_q[pb];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) qb.v[c] = s_q[(1 + c) * PN + pb];
+    qb.beta = s_q[4 * PN + pb];
+    qb.kin = s_q[5 * PN + pb];
+    qb.p = 0.0;
+    qb.c = 0.0;
+    double nrm[3], f[NV];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if constexpr (NODAL) {
+        // (Ja_x + Ja_m) / 2 from the halved rows: halving is exact
+        nrm[c] = s_ja[(d * 3 + c) * PN + pa] + s_ja[(d * 3 + c) * PN + pb];
+      } else {
+        nrm[c] = d == 0 ? jm[0][c] : (d == 1 ? jm[1][c] : jm[2][c]);
+      }
+    }
+    flux_chandrashekar_exact(p.inv_gm1, q, qb, nrm, f);
+    const double w = DsT[xd * DS + m];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+#pragma clang fp contract(off)
+      acc[v] = acc[v] + w * f[v];
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const bool lower = idx[d] == 0, upper = idx[d] == N;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+#pragma clang fp contract(off)
+      const double tl = acc[v] - fs[d][v] * p.lhat0;
+      const double tu = acc[v] + fs[d][v] * p.lhatN;
+      acc[v] = lower ? tl : (upper ? tu : acc[v]);
+    }
+  }
+  const double* tot = acc;
+
+  // 5. the viscous part, -1/J and the stage combination.  1/J, the base
+  // state and the result are touched once per launch: non-temporal accesses
+  // keep them from displacing the neighbours' face traces in the L2.
+  const int64_t g = e * NV * NN + a;
+  const double fac = -ij;
+  double r[NV];
+  if (p.visc) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+#pragma clang fp contract(off)
+      r[v] = (tot[v] + p.visc[g + v * NN]) * fac;
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) r[v] = tot[v] * fac;
+  }
+  if (p.mode == kModeRhs) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) __builtin_nontemporal_store(r[v], &p.out[g + v * NN]);
+  } else {
+    // the stage combinations of SSPRK3.hpp:12-143, rounded term by term
+    double s[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+#pragma clang fp contract(off)
+      s[v] = p.mode == kModeStage2 ? p.c1 * u[v] + p.c2 * r[v]
+                                   : p.c1 * __builtin_nontemporal_load(&p.u_base[g + v * NN]) +
+                                         p.c2 * u[v] + p.c3 * r[v];
+    }
+    if (p.floor_value > 0.0) {
+      if (s[0] < p.floor_value) s[0] = p.floor_value;
+      double m2 = 0.0;
+      for (int k = 0; k < DIM; ++k) m2 += s[k + 1] * s[k + 1];
+      const double kin = 0.5 * m2 / s[0];
+      if ((p.gamma - 1.0) * (s[NV - 1] - kin) < p.floor_value)
+        s[NV - 1] = p.floor_value / (p.gamma - 1.0) + kin;
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) __builtin_nontemporal_store(s[v], &p.out[g + v * NN]);
+  }
+}
+
+template <int N, bool NODAL, int SF>
+hipError_t launch_node_kernel(const KParams& p, hipStream_t stream) {
+  constexpr int NN = (N + 1) * (N + 1) * (N + 1);
+  if (p.E <= 0) return hipSuccess;
+  if (p.E > INT32_MAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((rhs_node_kernel<N, NODAL, SF>), dim3((unsigned)p.E), dim3(NN), 0, stream, p);
+  return hipGetLastError();
+}
+
+}  // namespace detail
+}  // namespace dgsem

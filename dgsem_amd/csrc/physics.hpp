@@ -218,11 +218,13 @@ DG_HD NodePrims<DIM> node_prims(double gamma, const double* u) {
 // The two logarithmic means (math_utils.hpp:8-24), the mean pressure
 // 0.5 rho_avg / beta_avg and 1 / beta_mean need five quotients; in the usual
 // case where both means take the series branch (f2 < 1e-4) they come from
-// two reciprocals:  1/(b_rho b_beta) gives f2 of both means, and
-// 1/(poly_rho s_beta) gives rho_mean = s_rho/poly_rho, 1/beta_mean =
-// poly_beta/s_beta and p_mean = s_rho/(2 s_beta), where s = x + y and
-// b = x(x+2y) + y^2.  Results agree with the quotient form to a few ulp.
-// Otherwise the means are evaluated exactly as the reference does.
+// two reciprocals:  1/(s_rho^2 s_beta^2) gives f2 = (x - y)^2 / s^2 of both
+// means, and 1/(poly_rho s_beta) gives rho_mean = s_rho/poly_rho,
+// 1/beta_mean = poly_beta/s_beta and p_mean = s_rho/(2 s_beta), where
+// s = x + y.  Results agree with the quotient form to a few ulp.  Otherwise
+// the means are the reference's (y - x) / log(y / x) with x <= y.
+// Every expression is symmetric in the two nodes, so f(a, b) and f(b, a)
+// are the same bits: a kernel may evaluate a pair from either side.
 template <int DIM>
 DG_HD void flux_chandrashekar(double inv_gm1, const NodePrims<DIM>& a, const NodePrims<DIM>& b,
                               const double* n, double* f) {
@@ -232,10 +234,9 @@ DG_HD void flux_chandrashekar(double inv_gm1, const NodePrims<DIM>& a, const Nod
     vnb += b.v[k] * n[k];
   }
   const double sr = a.rho + b.rho, sb = a.beta + b.beta;
-  const double nr = a.rho * (a.rho - 2.0 * b.rho) + b.rho * b.rho;
-  const double dr = a.rho * (a.rho + 2.0 * b.rho) + b.rho * b.rho;
-  const double nb = a.beta * (a.beta - 2.0 * b.beta) + b.beta * b.beta;
-  const double db = a.beta * (a.beta + 2.0 * b.beta) + b.beta * b.beta;
+  const double xr = a.rho - b.rho, xb = a.beta - b.beta;
+  const double nr = xr * xr, dr = sr * sr;
+  const double nb = xb * xb, db = sb * sb;
   const double r1 = recip(dr * db);
   const double f2r = nr * (db * r1), f2b = nb * (dr * r1);
   const double polyr = 2.0 + f2r * (2.0 / 3.0 + f2r * (2.0 / 5.0 + f2r * (2.0 / 7.0)));
@@ -247,8 +248,10 @@ DG_HD void flux_chandrashekar(double inv_gm1, const NodePrims<DIM>& a, const Nod
     inv_beta_mean = polyb * (polyr * r2);
     p_mean = 0.5 * sr * (polyr * r2);
   } else {
-    rho_mean = f2r < 1.0e-4 ? sr / polyr : (b.rho - a.rho) / log(b.rho / a.rho);
-    inv_beta_mean = f2b < 1.0e-4 ? polyb / sb : log(b.beta / a.beta) / (b.beta - a.beta);
+    const double rlo = dmin(a.rho, b.rho), rhi = dmax(a.rho, b.rho);
+    const double blo = dmin(a.beta, b.beta), bhi = dmax(a.beta, b.beta);
+    rho_mean = f2r < 1.0e-4 ? sr / polyr : (rhi - rlo) / log(rhi / rlo);
+    inv_beta_mean = f2b < 1.0e-4 ? polyb / sb : log(bhi / blo) / (bhi - blo);
     p_mean = 0.5 * sr / sb;
   }
   const double vsq_avg = a.kin + b.kin;

@@ -100,10 +100,9 @@ struct ChandrashekarMeans {
 template <int DIM>
 DG_HD ChandrashekarMeans chandrashekar_means(const NodePrims<DIM>& a, const NodePrims<DIM>& b) {
   const double sr = a.rho + b.rho, sb = a.beta + b.beta;
-  const double nr = a.rho * (a.rho - 2.0 * b.rho) + b.rho * b.rho;
-  const double dr = a.rho * (a.rho + 2.0 * b.rho) + b.rho * b.rho;
-  const double nb = a.beta * (a.beta - 2.0 * b.beta) + b.beta * b.beta;
-  const double db = a.beta * (a.beta + 2.0 * b.beta) + b.beta * b.beta;
+  const double xr = a.rho - b.rho, xb = a.beta - b.beta;
+  const double nr = xr * xr, dr = sr * sr;
+  const double nb = xb * xb, db = sb * sb;
   const double r1 = 1.0 / (dr * db);
   const double f2r = nr * (db * r1), f2b = nb * (dr * r1);
   ChandrashekarMeans m;
