@@ -265,23 +265,6 @@ __global__ void __launch_bounds__((N + 1) * (N + 1) * (N + 1)) __attribute__((am
     for (int c = 0; c < 3; ++c)
       jm[d][c] = NODAL ? p.Ja[((e * 3 + d) * 3 + c) * NN + a] : p.Ja[(e * 3 + d) * 3 + c];
   const double ij = NODAL ? p.invJ[e * NN + a] : p.invJ[e];
-  double nb_u[3][NV], nb_ja[3][3], nb_ij[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    int64_t lo, hi;
-    face_neighbours<DIM, false>(p, p.cells[0], p.cells[1], p.cells[2], e, d, lo, hi);
-    const bool lower = idx[d] == 0, upper = idx[d] == N;
-    const int64_t ne = lower ? lo : (upper ? hi : e);
-    const int nn = lower ? a + N * stride[d] : (upper ? a - N * stride[d] : a);
-#pragma unroll
-    for (int v = 0; v < NV; ++v) nb_u[d][v] = p.u[(ne * NV + v) * NN + nn];
-    const int64_t je = upper ? hi : e;
-    const int jn = upper ? nn : a;
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      nb_ja[d][c] = NODAL ? p.Ja[((je * 3 + d) * 3 + c) * NN + jn] : p.Ja[(je * 3 + d) * 3 + c];
-    nb_ij[d] = NODAL ? p.invJ[je * NN + jn] : p.invJ[je];
-  }
 
   // 2. the node's primitives and halved metric rows into LDS
   const NodePrims<DIM> q = node_prims_exact(p.gamma, u);
@@ -298,33 +281,6 @@ __global__ void __launch_bounds__((N + 1) * (N + 1) * (N + 1)) __attribute__((am
   }
   if (a < n * n) s_D[(a / n) * DS + a % n] = p.Ds[a / n][a % n];
   __syncthreads();
-
-  // 3. numerical fluxes on the faces this node lies on, lifted after the
-  // volume terms below
-  const double own_sg = ij >= 0.0 ? 1.0 : -1.0;
-  double fs[3][NV];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-#pragma unroll
-    for (int v = 0; v < NV; ++v) fs[d][v] = 0.0;
-    const bool lower = idx[d] == 0, upper = idx[d] == N;
-    if (lower || upper) {
-      const double nb_sg = nb_ij[d] >= 0.0 ? 1.0 : -1.0;
-      double ul[NV], ur[NV], nrm[3];
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        ul[v] = lower ? nb_u[d][v] : u[v];
-        ur[v] = lower ? u[v] : nb_u[d][v];
-      }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) nrm[c] = lower ? own_sg * jm[d][c] : nb_sg * nb_ja[d][c];
-      if constexpr (SF == kFluxLLF) {
-        flux_llf_exact(p.gamma, ul, ur, nrm, fs[d]);
-      } else {
-        flux_hllc_exact(p.gamma, ul, ur, nrm, fs[d]);
-      }
-    }
-  }
 
   // 4. flux differencing.  The reference (volume_flux.hpp:296-410) visits
   // the nodes in memory order and adds each pair's two terms as it goes, so
@@ -389,15 +345,51 @@ __global__ void __launch_bounds__((N + 1) * (N + 1) * (N + 1)) __attribute__((am
   // partners above on the z-line
 #pragma unroll 1
   for (int m = iz + 1; m <= N; ++m) term(2, iz, n * n, m, jz);
+  // 5. numerical fluxes on the faces this node lies on, lifted after the
+  // volume terms in direction order.  Their inputs are loaded here, not at
+  // the top, so that nothing but the sums lives through the term loops (a
+  // value held across them would spill).  The lower face takes the lower
+  // neighbour's node with index N, the upper face the upper neighbour's node
+  // with index 0 and its metric row and 1/J, whose sign and row make the
+  // interface normal; interior nodes load nothing.
+  const double own_sg = ij >= 0.0 ? 1.0 : -1.0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const bool lower = idx[d] == 0, upper = idx[d] == N;
+    if (lower || upper) {
+      int64_t lo, hi;
+      face_neighbours<DIM, false>(p, p.cells[0], p.cells[1], p.cells[2], e, d, lo, hi);
+      const int64_t ne = lower ? lo : hi;
+      const int nn = lower ? a + N * stride[d] : a - N * stride[d];
+      double un[NV], uo[NV], nrm[3], f[NV];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
+      for (int v = 0; v < NV; ++v) {
+        un[v] = p.u[(ne * NV + v) * NN + nn];
+        uo[v] = p.u[(e * NV + v) * NN + a];
+      }
+      if (lower) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          nrm[c] = own_sg * (NODAL ? 2.0 * s_ja[(d * 3 + c) * PN + pa] : jm[d][c]);
+      } else {
+        const double nij = NODAL ? p.invJ[ne * NN + nn] : p.invJ[ne];
+        const double nb_sg = nij >= 0.0 ? 1.0 : -1.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          nrm[c] = nb_sg * (NODAL ? p.Ja[((ne * 3 + d) * 3 + c) * NN + nn] : p.Ja[(ne * 3 + d) * 3 + c]);
+      }
+      const double* ul = lower ? un : uo;
+      const double* ur = lower ? uo : un;
+      if constexpr (SF == kFluxLLF) {
+        flux_llf_exact(p.gamma, ul, ur, nrm, f);
+      } else {
+        flux_hllc_exact(p.gamma, ul, ur, nrm, f);
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
 #pragma clang fp contract(off)
-      const double tl = acc[v] - fs[d][v] * p.lhat0;
-      const double tu = acc[v] + fs[d][v] * p.lhatN;
-      acc[v] = lower ? tl : (upper ? tu : acc[v]);
+        acc[v] = lower ? acc[v] - f[v] * p.lhat0 : acc[v] + f[v] * p.lhatN;
+      }
     }
   }
   const double* tot = acc;
@@ -407,6 +399,9 @@ __global__ void __launch_bounds__((N + 1) * (N + 1) * (N + 1)) __attribute__((am
   // keep them from displacing the neighbours' face traces in the L2.
   const int64_t g = e * NV * NN + a;
   const double fac = -ij;
+  double uk[NV];  // the stage input, reloaded (an L2 hit) rather than held
+#pragma unroll
+  for (int v = 0; v < NV; ++v) uk[v] = p.u[g + v * NN];
   double r[NV];
   if (p.visc) {
 #pragma unroll
@@ -427,9 +422,9 @@ __global__ void __launch_bounds__((N + 1) * (N + 1) * (N + 1)) __attribute__((am
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
 #pragma clang fp contract(off)
-      s[v] = p.mode == kModeStage2 ? p.c1 * u[v] + p.c2 * r[v]
+      s[v] = p.mode == kModeStage2 ? p.c1 * uk[v] + p.c2 * r[v]
                                    : p.c1 * __builtin_nontemporal_load(&p.u_base[g + v * NN]) +
-                                         p.c2 * u[v] + p.c3 * r[v];
+                                         p.c2 * uk[v] + p.c3 * r[v];
     }
     if (p.floor_value > 0.0) {
       if (s[0] < p.floor_value) s[0] = p.floor_value;
