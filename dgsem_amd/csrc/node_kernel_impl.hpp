@@ -249,12 +249,8 @@ __global__ void __launch_bounds__((N + 1) * (N + 1) * (N + 1)) __attribute__((am
   constexpr int stride[3] = {1, n, n * n};
   const int pa = lds_pos<N>(a);
 
-  // 1. loads: the node's state, metric rows and 1/J; per direction, the face
-  // neighbour's trace facing this node (lower face: the lower neighbour's
-  // node with index N; upper face: the upper neighbour's node with index 0,
-  // and its metric row and 1/J, whose sign and row make the interface
-  // normal).  Interior nodes read their own node instead, so every load is
-  // unconditional and hits lines the block reads anyway.
+  // 1. the node's state, metric rows and 1/J (the face data follows the
+  // term loops, phase 4)
   double u[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) u[v] = p.u[(e * NV + v) * NN + a];
@@ -282,7 +278,7 @@ __global__ void __launch_bounds__((N + 1) * (N + 1) * (N + 1)) __attribute__((am
   if (a < n * n) s_D[(a / n) * DS + a % n] = p.Ds[a / n][a % n];
   __syncthreads();
 
-  // 4. flux differencing.  The reference (volume_flux.hpp:296-410) visits
+  // 3. flux differencing.  The reference (volume_flux.hpp:296-410) visits
   // the nodes in memory order and adds each pair's two terms as it goes, so
   // a node's sum takes its terms in the order: partners below it on its
   // z-line, y-line and x-line, then partners above it on its x-, y- and
@@ -345,7 +341,7 @@ __global__ void __launch_bounds__((N + 1) * (N + 1) * (N + 1)) __attribute__((am
   // partners above on the z-line
 #pragma unroll 1
   for (int m = iz + 1; m <= N; ++m) term(2, iz, n * n, m, jz);
-  // 5. numerical fluxes on the faces this node lies on, lifted after the
+  // 4. numerical fluxes on the faces this node lies on, lifted after the
   // volume terms in direction order.  Their inputs are loaded here, not at
   // the top, so that nothing but the sums lives through the term loops (a
   // value held across them would spill).  The lower face takes the lower
