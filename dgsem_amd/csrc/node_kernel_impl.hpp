@@ -395,6 +395,14 @@ __global__ void __launch_bounds__((N + 1) * (N + 1) * (N + 1)) __attribute__((am
   // keep them from displacing the neighbours' face traces in the L2.
   const int64_t g = e * NV * NN + a;
   const double fac = -ij;
+  // the base state's loads go out together (a select between the stage
+  // kinds inside the variable loop would serialise them)
+  const bool with_base = p.mode == kModeStage3;
+  double ub[NV];
+  if (with_base) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) ub[v] = __builtin_nontemporal_load(&p.u_base[g + v * NN]);
+  }
   double uk[NV];  // the stage input, reloaded (an L2 hit) rather than held
 #pragma unroll
   for (int v = 0; v < NV; ++v) uk[v] = p.u[g + v * NN];
@@ -415,12 +423,18 @@ __global__ void __launch_bounds__((N + 1) * (N + 1) * (N + 1)) __attribute__((am
   } else {
     // the stage combinations of SSPRK3.hpp:12-143, rounded term by term
     double s[NV];
+    if (with_base) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
+      for (int v = 0; v < NV; ++v) {
 #pragma clang fp contract(off)
-      s[v] = p.mode == kModeStage2 ? p.c1 * uk[v] + p.c2 * r[v]
-                                   : p.c1 * __builtin_nontemporal_load(&p.u_base[g + v * NN]) +
-                                         p.c2 * uk[v] + p.c3 * r[v];
+        s[v] = p.c1 * ub[v] + p.c2 * uk[v] + p.c3 * r[v];
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+#pragma clang fp contract(off)
+        s[v] = p.c1 * uk[v] + p.c2 * r[v];
+      }
     }
     if (p.floor_value > 0.0) {
       if (s[0] < p.floor_value) s[0] = p.floor_value;
