@@ -83,12 +83,12 @@ def test_navier_stokes_n7_rhs(gpu, mu):
 
 @pytest.mark.parametrize("geometry", [dg.GEOMETRY_NODAL, dg.GEOMETRY_AFFINE])
 def test_tgv_n7_ssprk3_steps(gpu, geometry):
-    print(f"case tgv N=7 steps geometry={geometry}", flush=True)
+    print(f"case tgv N=7 100 steps geometry={geometry}", flush=True)
     oc, gs = make_pair(3, 7, (2, 2, 2), [0, 0, 0], [TWO_PI] * 3, geometry=geometry)
     u0 = oc.init_tgv()
     dt = 2e-4
-    ref = oc.ssprk3(u0, dt, 20)
+    ref = oc.ssprk3(u0, dt, 100)
     u = gs.from_reference_layout(u0)
     t1, t2 = gs.empty(), gs.empty()
-    gs.ssprk3_steps(u, dt, 20, t1, t2)
+    gs.ssprk3_steps(u, dt, 100, t1, t2)
     assert rel_l2(gs.to_reference_layout(u), ref) <= STEPS_TOL
