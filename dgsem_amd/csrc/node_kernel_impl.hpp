@@ -304,7 +304,7 @@ __global__ void __launch_bounds__((N + 1) * (N + 1) * (N + 1)) __attribute__((am
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       // NODAL: (Ja_x + Ja_m) / 2 from the halved rows (halving is exact)
-      nrm[c] = NODAL ? s_ja[(d * 3 + c) * PN + pa] + s_ja[(d * 3 + c) * PN + pb] : jown[c];
+      nrm[c] = NODAL ? jown[c] + s_ja[(d * 3 + c) * PN + pb] : jown[c];
     }
     flux_chandrashekar_exact(p.inv_gm1, q, qb, nrm, f);
     const double w = s_D[xd * DS + m];
@@ -314,14 +314,13 @@ __global__ void __launch_bounds__((N + 1) * (N + 1) * (N + 1)) __attribute__((am
       acc[v] = acc[v] + w * f[v];
     }
   };
-  // affine: the element's rows (NODAL reads the node's halved rows from LDS
-  // in each term, which keeps them out of registers)
+  // the node's own rows (NODAL: halved, from LDS; affine: the element's)
   double jx[3], jy[3], jz[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    jx[c] = NODAL ? 0.0 : jm[0][c];
-    jy[c] = NODAL ? 0.0 : jm[1][c];
-    jz[c] = NODAL ? 0.0 : jm[2][c];
+    jx[c] = NODAL ? s_ja[(0 * 3 + c) * PN + pa] : jm[0][c];
+    jy[c] = NODAL ? s_ja[(1 * 3 + c) * PN + pa] : jm[1][c];
+    jz[c] = NODAL ? s_ja[(2 * 3 + c) * PN + pa] : jm[2][c];
   }
   // partners below on the z-line
 #pragma unroll 1
