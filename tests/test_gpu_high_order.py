@@ -44,7 +44,7 @@ def test_tgv_n7_rhs(gpu, geometry, surface):
     assert rel_l2(gpu_rhs(gs, u), oc.rhs(u)) <= RHS_TOL
 
 
-@pytest.mark.parametrize("N,cells", [(7, (2, 2, 2)), (3, (3, 3, 2))])
+@pytest.mark.parametrize("N,cells", [(7, (2, 2, 2)), (7, (1, 1, 2)), (3, (3, 3, 2))])
 def test_rough_state_rhs_and_conservation(gpu, N, cells):
     print(f"case rough N={N} cells={cells}", flush=True)
     oc, gs = make_pair(3, N, cells, [0, 0, 0], [1, 1, 1])
