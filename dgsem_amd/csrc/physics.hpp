@@ -126,7 +126,7 @@ DG_HD void flux_llf(double gamma, const double* ul, const double* ur, const doub
 template <int DIM>
 DG_HD void flux_hllc(double gamma, const double* ul, const double* ur, const double* n, double* f) {
   constexpr int NV = DIM + 2;
-  double nrm = 0.0, nh[DIM], vl[DIM], vr[DIM], fl[NV], fr[NV];
+  double nrm = 0.0, nh[DIM], vl[DIM], vr[DIM];
   for (int d = 0; d < DIM; ++d) nrm += n[d] * n[d];
   nrm = sqrt(nrm);
   for (int d = 0; d < DIM; ++d) nh[d] = n[d] / nrm;
@@ -144,8 +144,6 @@ DG_HD void flux_hllc(double gamma, const double* ul, const double* ur, const dou
   const double pl = (gamma - 1.0) * (El - 0.5 * rl * v2l);
   const double pr = (gamma - 1.0) * (Er - 0.5 * rr * v2r);
   const double cl = sqrt(gamma * pl / rl), cr = sqrt(gamma * pr / rr);
-  physical_flux_n<DIM>(gamma, ul, nh, fl);
-  physical_flux_n<DIM>(gamma, ur, nh, fr);
   const double sl = sqrt(rl), sr = sqrt(rr), ss = sl + sr;
   double vroe2 = 0.0;
   for (int d = 0; d < DIM; ++d) {
@@ -158,16 +156,18 @@ DG_HD void flux_hllc(double gamma, const double* ul, const double* ur, const dou
   const double croe = sqrt((gamma - 1.0) * (Hroe - 0.5 * vroe2));
   const double SL = dmin(vnl - cl, vroe - croe);
   const double SR = dmax(vnr + cr, vroe + croe);
-  if (SL >= 0) {
-    for (int i = 0; i < NV; ++i) f[i] = fl[i];
-  } else if (SR <= 0) {
-    for (int i = 0; i < NV; ++i) f[i] = fr[i];
+  const double mL = SL - vnl, mR = SR - vnr;
+  // the side whose state (or star state) gives the flux; only that side's
+  // physical flux is formed
+  const bool star = !(SL >= 0) && !(SR <= 0);
+  const double Ss = star ? (pr - pl + rl * vnl * mL - rr * vnr * mR) / (rl * mL - rr * mR) : 0.0;
+  const bool left = star ? Ss >= 0 : SL >= 0;
+  const double* uk = left ? ul : ur;
+  double fk[NV];
+  physical_flux_n<DIM>(gamma, uk, nh, fk);
+  if (!star) {
+    for (int i = 0; i < NV; ++i) f[i] = fk[i];
   } else {
-    const double mL = SL - vnl, mR = SR - vnr;
-    const double Ss = (pr - pl + rl * vnl * mL - rr * vnr * mR) / (rl * mL - rr * mR);
-    const bool left = Ss >= 0;
-    const double* uk = left ? ul : ur;
-    const double* fk = left ? fl : fr;
     const double* vk = left ? vl : vr;
     const double rk = left ? rl : rr, vnk = left ? vnl : vnr, ek = left ? el : er;
     const double pk = left ? pl : pr, Sk = left ? SL : SR, mk = left ? mL : mR;
