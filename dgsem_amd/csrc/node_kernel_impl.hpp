@@ -1,10 +1,11 @@
-// Node-parallel variant of the fused RHS / SSPRK3 stage kernel for 3D
-// split-form elements of high polynomial degree (periodic meshes, no shock
-// capturing).
+// Node-parallel variant of the fused RHS / SSPRK3 stage kernel for
+// split-form elements of high polynomial degree, 2D and 3D (periodic
+// meshes, no shock capturing).
 //
 // The line-thread kernel (rhs_kernel) keeps all N+1 node sums of a line in
-// registers; at N = 7 that is 40 accumulators plus the line's metric rows,
-// which takes the kernel past 400 registers and down to one wave per SIMD.
+// registers; at N = 7 in 3D that is 40 accumulators plus the line's metric
+// rows, which takes the kernel past 400 registers and down to one wave per
+// SIMD.
 // Here one workgroup owns one element and one thread owns one node: it
 // walks the three lines through its node and sums D_split(x, m) f(u_x, u_m)
 // over the partners m, evaluating every pair flux itself.  Each pair is thus
@@ -42,10 +43,11 @@ namespace detail {
 #define DGSEM_NODE_KERNEL_MIN_N 7
 #endif
 
-// 3D split-form elements of degree N run the node-parallel kernel.
+// Split-form elements of degree N >= 7 on periodic meshes run the
+// node-parallel kernel, in 2D and 3D.
 template <int DIM, int N, bool SC, bool BCS>
 constexpr bool use_node_kernel() {
-  return DIM == 3 && !SC && !BCS && N >= DGSEM_NODE_KERNEL_MIN_N;
+  return !SC && !BCS && N >= DGSEM_NODE_KERNEL_MIN_N;
 }
 
 // The functions below round every operation as the reference's scalar code
@@ -71,17 +73,22 @@ __device__ __forceinline__ double div_rn(double a, double b) {
   return fma(fma(-b, q, a), y, q);
 }
 
+// Sums below run left to right from their first term, as the reference's
+// written-out expressions do ((a + b) + c), in 2D and 3D alike.
+
 // rho, v, beta = rho / (2 p), |v|^2 / 2, p and c of a conservative state.
-__device__ __forceinline__ NodePrims<3> node_prims_exact(double g, const double* u) {
+template <int DIM>
+__device__ __forceinline__ NodePrims<DIM> node_prims_exact(double g, const double* u) {
 #pragma clang fp contract(off)
-  NodePrims<3> q;
+  NodePrims<DIM> q;
   q.rho = u[0];
-  q.v[0] = u[1] / u[0];
-  q.v[1] = u[2] / u[0];
-  q.v[2] = u[3] / u[0];
-  const double v2 = q.v[0] * q.v[0] + q.v[1] * q.v[1] + q.v[2] * q.v[2];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) q.v[k] = u[k + 1] / u[0];
+  double v2 = q.v[0] * q.v[0];
+#pragma unroll
+  for (int k = 1; k < DIM; ++k) v2 = v2 + q.v[k] * q.v[k];
   q.kin = 0.5 * v2;
-  q.p = (g - 1.0) * (u[4] - 0.5 * u[0] * v2);
+  q.p = (g - 1.0) * (u[DIM + 1] - 0.5 * u[0] * v2);
   q.beta = 0.5 * u[0] / q.p;
   q.c = sqrt(g * q.p / u[0]);
   return q;
@@ -90,12 +97,17 @@ __device__ __forceinline__ NodePrims<3> node_prims_exact(double g, const double*
 // Chandrashekar flux, symmetric in (a, b) to the bit: the series test uses
 // ((x - y) / (x + y))^2 (whose rounding cannot reach the polynomial, which
 // is 2 to within 1e-4), and the log branch orders its arguments by value.
-__device__ __forceinline__ void flux_chandrashekar_exact(double inv_gm1, const NodePrims<3>& a,
-                                                         const NodePrims<3>& b, const double* n,
+template <int DIM>
+__device__ __forceinline__ void flux_chandrashekar_exact(double inv_gm1, const NodePrims<DIM>& a,
+                                                         const NodePrims<DIM>& b, const double* n,
                                                          double* f) {
 #pragma clang fp contract(off)
-  const double vna = a.v[0] * n[0] + a.v[1] * n[1] + a.v[2] * n[2];
-  const double vnb = b.v[0] * n[0] + b.v[1] * n[1] + b.v[2] * n[2];
+  double vna = a.v[0] * n[0], vnb = b.v[0] * n[0];
+#pragma unroll
+  for (int k = 1; k < DIM; ++k) {
+    vna = vna + a.v[k] * n[k];
+    vnb = vnb + b.v[k] * n[k];
+  }
   const double sr = a.rho + b.rho, sb = a.beta + b.beta;
   const double xr = a.rho - b.rho, xb = a.beta - b.beta;
   const double r1 = recip((sr * sr) * (sb * sb));
@@ -119,62 +131,86 @@ __device__ __forceinline__ void flux_chandrashekar_exact(double inv_gm1, const N
   double fe = f1 * 0.5 * (div_rn(inv_gm1, beta_mean) - vsq);
   f[0] = f1;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < DIM; ++k) {
     const double vavg = 0.5 * (a.v[k] + b.v[k]);
     f[k + 1] = f1 * vavg + p_mean * n[k];
     fe = fe + f[k + 1] * vavg;
   }
-  f[4] = fe;
+  f[DIM + 1] = fe;
 }
 
-// n . (F_x, F_y, F_z)(u) (compressible_euler3D.hpp:26-67) and the state's
-// pressure.
+// n . (F_1, ..., F_DIM)(u) (compressible_euler2D.hpp:25-57,
+// compressible_euler3D.hpp:26-67) and the state's pressure.
+template <int DIM>
 __device__ __forceinline__ double physical_flux_exact(double g, const double* u, const double* n,
                                                       double* f) {
 #pragma clang fp contract(off)
-  const double v0 = u[1] / u[0], v1 = u[2] / u[0], v2 = u[3] / u[0];
-  const double p = (g - 1.0) * (u[4] - 0.5 * u[0] * (v0 * v0 + v1 * v1 + v2 * v2));
-  const double fx[5] = {u[1], u[1] * v0 + p, u[1] * v1, u[1] * v2, v0 * (u[4] + p)};
-  const double fy[5] = {u[2], u[2] * v0, u[2] * v1 + p, u[2] * v2, v1 * (u[4] + p)};
-  const double fz[5] = {u[3], u[3] * v0, u[3] * v1, u[3] * v2 + p, v2 * (u[4] + p)};
+  constexpr int NV = DIM + 2;
+  double v[DIM];
 #pragma unroll
-  for (int v = 0; v < 5; ++v) f[v] = n[0] * fx[v] + n[1] * fy[v] + n[2] * fz[v];
+  for (int k = 0; k < DIM; ++k) v[k] = u[k + 1] / u[0];
+  double v2 = v[0] * v[0];
+#pragma unroll
+  for (int k = 1; k < DIM; ++k) v2 = v2 + v[k] * v[k];
+  const double p = (g - 1.0) * (u[NV - 1] - 0.5 * u[0] * v2);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    double fd[NV];
+    fd[0] = u[d + 1];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) fd[k + 1] = k == d ? u[d + 1] * v[k] + p : u[d + 1] * v[k];
+    fd[NV - 1] = v[d] * (u[NV - 1] + p);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) f[i] = d == 0 ? n[0] * fd[i] : f[i] + n[d] * fd[i];
+  }
   return p;
 }
 
 // Local Lax-Friedrichs flux from conservative states (numerical_flux.hpp:54-68
-// with compressible_euler3D.hpp:121-166).
+// with compressible_euler{2D,3D}.hpp get_wave_speed).
+template <int DIM>
 __device__ __forceinline__ void flux_llf_exact(double g, const double* ul, const double* ur,
                                                const double* n, double* f) {
 #pragma clang fp contract(off)
-  double fl[5], fr[5], vn[2], a[2];
+  constexpr int NV = DIM + 2;
+  double fl[NV], fr[NV], vn[2], a[2];
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
     const double* u = s == 0 ? ul : ur;
-    const double p = physical_flux_exact(g, u, n, s == 0 ? fl : fr);
+    const double p = physical_flux_exact<DIM>(g, u, n, s == 0 ? fl : fr);
     a[s] = sqrt(g * p / u[0]);
-    vn[s] = (u[1] / u[0]) * n[0] + (u[2] / u[0]) * n[1] + (u[3] / u[0]) * n[2];
-  }
-  const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-  const double lam = dmax(fabs(vn[0]), fabs(vn[1])) + dmax(a[0], a[1]) * nn;
+    double vnn = (u[1] / u[0]) * n[0];
 #pragma unroll
-  for (int v = 0; v < 5; ++v) f[v] = 0.5 * (fl[v] + fr[v]) - 0.5 * lam * (ur[v] - ul[v]);
+    for (int k = 1; k < DIM; ++k) vnn = vnn + (u[k + 1] / u[0]) * n[k];
+    vn[s] = vnn;
+  }
+  double nn2 = n[0] * n[0];
+#pragma unroll
+  for (int k = 1; k < DIM; ++k) nn2 = nn2 + n[k] * n[k];
+  const double lam = dmax(fabs(vn[0]), fabs(vn[1])) + dmax(a[0], a[1]) * sqrt(nn2);
+#pragma unroll
+  for (int v = 0; v < NV; ++v) f[v] = 0.5 * (fl[v] + fr[v]) - 0.5 * lam * (ur[v] - ul[v]);
 }
 
 // HLLC flux with a normal vector (a build extension: the reference has
 // HLLC for 1D Euler only, numerical_flux.hpp:283-378, whose structure this
 // follows in the frame of the unit normal, scaled by |n|).
+template <int DIM>
 __device__ __forceinline__ void flux_hllc_exact(double g, const double* ul, const double* ur,
                                                 const double* n, double* f) {
 #pragma clang fp contract(off)
-  double nh[3], vl[3], vr[3], fl[5], fr[5];
-  const double nrm = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  constexpr int NV = DIM + 2;
+  double nh[DIM], vl[DIM], vr[DIM], fl[NV], fr[NV];
+  double nn2 = n[0] * n[0];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) nh[d] = n[d] / nrm;
-  const double rho_ll = ul[0], rho_rr = ur[0], rhoE_ll = ul[4], rhoE_rr = ur[4];
+  for (int k = 1; k < DIM; ++k) nn2 = nn2 + n[k] * n[k];
+  const double nrm = sqrt(nn2);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) nh[d] = n[d] / nrm;
+  const double rho_ll = ul[0], rho_rr = ur[0], rhoE_ll = ul[NV - 1], rhoE_rr = ur[NV - 1];
   double v2l = 0.0, v2r = 0.0, vn_l = 0.0, vn_r = 0.0;
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
+  for (int d = 0; d < DIM; ++d) {
     vl[d] = ul[d + 1] / rho_ll;
     vr[d] = ur[d + 1] / rho_rr;
     v2l += vl[d] * vl[d];
@@ -186,12 +222,12 @@ __device__ __forceinline__ void flux_hllc_exact(double g, const double* ul, cons
   const double p_ll = (g - 1.0) * (rhoE_ll - 0.5 * rho_ll * v2l);
   const double p_rr = (g - 1.0) * (rhoE_rr - 0.5 * rho_rr * v2r);
   const double c_ll = sqrt(g * p_ll / rho_ll), c_rr = sqrt(g * p_rr / rho_rr);
-  physical_flux_exact(g, ul, nh, fl);
-  physical_flux_exact(g, ur, nh, fr);
+  physical_flux_exact<DIM>(g, ul, nh, fl);
+  physical_flux_exact<DIM>(g, ur, nh, fr);
   const double srl = sqrt(rho_ll), srr = sqrt(rho_rr), ssum = srl + srr;
   double vroe2 = 0.0;
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
+  for (int d = 0; d < DIM; ++d) {
     const double vroe_d = (srl * vl[d] + srr * vr[d]) / ssum;
     vroe2 += vroe_d * vroe_d;
   }
@@ -205,10 +241,10 @@ __device__ __forceinline__ void flux_hllc_exact(double g, const double* ul, cons
   const double sMu_L = Ssl - vn_l, sMu_R = Ssr - vn_r;
   if (Ssl >= 0) {
 #pragma unroll
-    for (int i = 0; i < 5; ++i) f[i] = fl[i];
+    for (int i = 0; i < NV; ++i) f[i] = fl[i];
   } else if (Ssr <= 0) {
 #pragma unroll
-    for (int i = 0; i < 5; ++i) f[i] = fr[i];
+    for (int i = 0; i < NV; ++i) f[i] = fr[i];
   } else {
     const double SStar = (p_rr - p_ll + rho_ll * vn_l * sMu_L - rho_rr * vn_r * sMu_R) /
                          (rho_ll * sMu_L - rho_rr * sMu_R);
@@ -223,57 +259,59 @@ __device__ __forceinline__ void flux_hllc_exact(double g, const double* ul, cons
     const double enerStar = Ek + (SStar - vnk) * (SStar + pk / (rk * sMu));
     f[0] = fk[0] + Sk * (densStar - uk[0]);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
+    for (int d = 0; d < DIM; ++d) {
       const double mstar = densStar * (vk[d] + (SStar - vnk) * nh[d]);
       f[d + 1] = fk[d + 1] + Sk * (mstar - uk[d + 1]);
     }
-    f[4] = fk[4] + Sk * (densStar * enerStar - uk[4]);
+    f[NV - 1] = fk[NV - 1] + Sk * (densStar * enerStar - uk[NV - 1]);
   }
 #pragma unroll
-  for (int i = 0; i < 5; ++i) f[i] *= nrm;
+  for (int i = 0; i < NV; ++i) f[i] *= nrm;
 }
 
-template <int N, bool NODAL, int SF>
-__global__ void __launch_bounds__((N + 1) * (N + 1) * (N + 1)) __attribute__((amdgpu_waves_per_eu(4)))
+template <int DIM, int N, bool NODAL, int SF>
+__global__ void __launch_bounds__((Shape<DIM, N>::NN)) __attribute__((amdgpu_waves_per_eu(4)))
     rhs_node_kernel(const KParams p) {
-  constexpr int DIM = 3, n = N + 1, NV = 5, NN = n * n * n;
+  constexpr int n = N + 1, NV = DIM + 2, NN = Shape<DIM, N>::NN;
   constexpr int PN = NN + NN / n;  // padded plane (lds_pos)
   constexpr int DS = n + 1;        // padded row of the D_split table
-  __shared__ double s_q[6 * PN];                 // rho, v[3], beta, kin
-  __shared__ double s_ja[NODAL ? 9 * PN : 1];    // Ja^d_c / 2
+  constexpr int QB = DIM + 1;      // plane of beta; kin follows
+  __shared__ double s_q[(DIM + 3) * PN];             // rho, v[DIM], beta, kin
+  __shared__ double s_ja[NODAL ? DIM * DIM * PN : 1];  // Ja^d_c / 2
   __shared__ double s_D[n * DS];
 
   const int a = threadIdx.x;
   const int64_t e = xcd_block(blockIdx.x, gridDim.x);
-  const int idx[3] = {a % n, (a / n) % n, a / (n * n)};
+  const int idx[3] = {a % n, (a / n) % n, DIM == 3 ? a / (n * n) : 0};
   constexpr int stride[3] = {1, n, n * n};
   const int pa = lds_pos<N>(a);
+  const int nz = DIM == 3 ? p.cells[2] : 1;
 
   // 1. the node's state, metric rows and 1/J (the face data follows the
   // term loops, phase 4)
   double u[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) u[v] = p.u[(e * NV + v) * NN + a];
-  double jm[3][3];
+  double jm[DIM][DIM];
 #pragma unroll
-  for (int d = 0; d < 3; ++d)
+  for (int d = 0; d < DIM; ++d)
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-      jm[d][c] = NODAL ? p.Ja[((e * 3 + d) * 3 + c) * NN + a] : p.Ja[(e * 3 + d) * 3 + c];
+    for (int c = 0; c < DIM; ++c)
+      jm[d][c] = NODAL ? p.Ja[((e * DIM + d) * DIM + c) * NN + a] : p.Ja[(e * DIM + d) * DIM + c];
   const double ij = NODAL ? p.invJ[e * NN + a] : p.invJ[e];
 
   // 2. the node's primitives and halved metric rows into LDS
-  const NodePrims<DIM> q = node_prims_exact(p.gamma, u);
+  const NodePrims<DIM> q = node_prims_exact<DIM>(p.gamma, u);
   s_q[pa] = q.rho;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) s_q[(1 + c) * PN + pa] = q.v[c];
-  s_q[4 * PN + pa] = q.beta;
-  s_q[5 * PN + pa] = q.kin;
+  for (int c = 0; c < DIM; ++c) s_q[(1 + c) * PN + pa] = q.v[c];
+  s_q[QB * PN + pa] = q.beta;
+  s_q[(QB + 1) * PN + pa] = q.kin;
   if constexpr (NODAL) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d)
+    for (int d = 0; d < DIM; ++d)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) s_ja[(d * 3 + c) * PN + pa] = 0.5 * jm[d][c];
+      for (int c = 0; c < DIM; ++c) s_ja[(d * DIM + c) * PN + pa] = 0.5 * jm[d][c];
   }
   if (a < n * n) s_D[(a / n) * DS + a % n] = p.Ds[a / n][a % n];
   __syncthreads();
@@ -295,18 +333,18 @@ __global__ void __launch_bounds__((N + 1) * (N + 1) * (N + 1)) __attribute__((am
     NodePrims<DIM> qb;
     qb.rho = s_q[pb];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) qb.v[c] = s_q[(1 + c) * PN + pb];
-    qb.beta = s_q[4 * PN + pb];
-    qb.kin = s_q[5 * PN + pb];
+    for (int c = 0; c < DIM; ++c) qb.v[c] = s_q[(1 + c) * PN + pb];
+    qb.beta = s_q[QB * PN + pb];
+    qb.kin = s_q[(QB + 1) * PN + pb];
     qb.p = 0.0;
     qb.c = 0.0;
-    double nrm[3], f[NV];
+    double nrm[DIM], f[NV];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < DIM; ++c) {
       // NODAL: (Ja_x + Ja_m) / 2 from the halved rows (halving is exact)
-      nrm[c] = NODAL ? jown[c] + s_ja[(d * 3 + c) * PN + pb] : jown[c];
+      nrm[c] = NODAL ? jown[c] + s_ja[(d * DIM + c) * PN + pb] : jown[c];
     }
-    flux_chandrashekar_exact(p.inv_gm1, q, qb, nrm, f);
+    flux_chandrashekar_exact<DIM>(p.inv_gm1, q, qb, nrm, f);
     const double w = s_D[xd * DS + m];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
@@ -315,16 +353,18 @@ __global__ void __launch_bounds__((N + 1) * (N + 1) * (N + 1)) __attribute__((am
     }
   };
   // the node's own rows (NODAL: halved, from LDS; affine: the element's)
-  double jx[3], jy[3], jz[3];
+  double jx[DIM], jy[DIM], jz[DIM];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    jx[c] = NODAL ? s_ja[(0 * 3 + c) * PN + pa] : jm[0][c];
-    jy[c] = NODAL ? s_ja[(1 * 3 + c) * PN + pa] : jm[1][c];
-    jz[c] = NODAL ? s_ja[(2 * 3 + c) * PN + pa] : jm[2][c];
+  for (int c = 0; c < DIM; ++c) {
+    jx[c] = NODAL ? s_ja[(0 * DIM + c) * PN + pa] : jm[0][c];
+    jy[c] = NODAL ? s_ja[(1 * DIM + c) * PN + pa] : jm[1][c];
+    jz[c] = DIM == 3 ? (NODAL ? s_ja[((DIM - 1) * DIM + c) * PN + pa] : jm[DIM - 1][c]) : 0.0;
   }
-  // partners below on the z-line
+  // partners below on the z-line (3D)
+  if constexpr (DIM == 3) {
 #pragma unroll 1
-  for (int m = 0; m < iz; ++m) term(2, iz, n * n, m, jz);
+    for (int m = 0; m < iz; ++m) term(2, iz, n * n, m, jz);
+  }
   // below on the y-line, below and above on the x-line, above on the y-line:
   // 2N terms for every node
 #pragma unroll 1
@@ -332,14 +372,16 @@ __global__ void __launch_bounds__((N + 1) * (N + 1) * (N + 1)) __attribute__((am
     const bool yl = t < iy, xl = t < iy + ix, xu = t < iy + N;
     const bool dy = yl || !xu;
     const int m = yl ? t : (xl ? t - iy : (xu ? t - iy + 1 : t - N + 1));
-    double jo[3];
+    double jo[DIM];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) jo[c] = dy ? jy[c] : jx[c];
+    for (int c = 0; c < DIM; ++c) jo[c] = dy ? jy[c] : jx[c];
     term(dy ? 1 : 0, dy ? iy : ix, dy ? n : 1, m, jo);
   }
-  // partners above on the z-line
+  // partners above on the z-line (3D)
+  if constexpr (DIM == 3) {
 #pragma unroll 1
-  for (int m = iz + 1; m <= N; ++m) term(2, iz, n * n, m, jz);
+    for (int m = iz + 1; m <= N; ++m) term(2, iz, n * n, m, jz);
+  }
   // 4. numerical fluxes on the faces this node lies on, lifted after the
   // volume terms in direction order.  Their inputs are loaded here, not at
   // the top, so that nothing but the sums lives through the term loops (a
@@ -349,14 +391,14 @@ __global__ void __launch_bounds__((N + 1) * (N + 1) * (N + 1)) __attribute__((am
   // interface normal; interior nodes load nothing.
   const double own_sg = ij >= 0.0 ? 1.0 : -1.0;
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
+  for (int d = 0; d < DIM; ++d) {
     const bool lower = idx[d] == 0, upper = idx[d] == N;
     if (lower || upper) {
       int64_t lo, hi;
-      face_neighbours<DIM, false>(p, p.cells[0], p.cells[1], p.cells[2], e, d, lo, hi);
+      face_neighbours<DIM, false>(p, p.cells[0], p.cells[1], nz, e, d, lo, hi);
       const int64_t ne = lower ? lo : hi;
       const int nn = lower ? a + N * stride[d] : a - N * stride[d];
-      double un[NV], uo[NV], nrm[3], f[NV];
+      double un[NV], uo[NV], nrm[DIM], f[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         un[v] = p.u[(ne * NV + v) * NN + nn];
@@ -364,21 +406,22 @@ __global__ void __launch_bounds__((N + 1) * (N + 1) * (N + 1)) __attribute__((am
       }
       if (lower) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-          nrm[c] = own_sg * (NODAL ? 2.0 * s_ja[(d * 3 + c) * PN + pa] : jm[d][c]);
+        for (int c = 0; c < DIM; ++c)
+          nrm[c] = own_sg * (NODAL ? 2.0 * s_ja[(d * DIM + c) * PN + pa] : jm[d][c]);
       } else {
         const double nij = NODAL ? p.invJ[ne * NN + nn] : p.invJ[ne];
         const double nb_sg = nij >= 0.0 ? 1.0 : -1.0;
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-          nrm[c] = nb_sg * (NODAL ? p.Ja[((ne * 3 + d) * 3 + c) * NN + nn] : p.Ja[(ne * 3 + d) * 3 + c]);
+        for (int c = 0; c < DIM; ++c)
+          nrm[c] = nb_sg * (NODAL ? p.Ja[((ne * DIM + d) * DIM + c) * NN + nn]
+                                  : p.Ja[(ne * DIM + d) * DIM + c]);
       }
       const double* ul = lower ? un : uo;
       const double* ur = lower ? uo : un;
       if constexpr (SF == kFluxLLF) {
-        flux_llf_exact(p.gamma, ul, ur, nrm, f);
+        flux_llf_exact<DIM>(p.gamma, ul, ur, nrm, f);
       } else {
-        flux_hllc_exact(p.gamma, ul, ur, nrm, f);
+        flux_hllc_exact<DIM>(p.gamma, ul, ur, nrm, f);
       }
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
@@ -448,12 +491,13 @@ __global__ void __launch_bounds__((N + 1) * (N + 1) * (N + 1)) __attribute__((am
   }
 }
 
-template <int N, bool NODAL, int SF>
+template <int DIM, int N, bool NODAL, int SF>
 hipError_t launch_node_kernel(const KParams& p, hipStream_t stream) {
-  constexpr int NN = (N + 1) * (N + 1) * (N + 1);
+  constexpr int NN = Shape<DIM, N>::NN;
   if (p.E <= 0) return hipSuccess;
   if (p.E > INT32_MAX) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((rhs_node_kernel<N, NODAL, SF>), dim3((unsigned)p.E), dim3(NN), 0, stream, p);
+  hipLaunchKernelGGL((rhs_node_kernel<DIM, N, NODAL, SF>), dim3((unsigned)p.E), dim3(NN), 0, stream,
+                     p);
   return hipGetLastError();
 }
 

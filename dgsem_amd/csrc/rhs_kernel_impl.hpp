@@ -309,7 +309,7 @@ namespace detail {
 
 template <int DIM, int N, bool SC, bool NODAL, int SF, int FF, bool BCS>
 hipError_t launch_one(const KParams& p, hipStream_t stream) {
-  if constexpr (use_node_kernel<DIM, N, SC, BCS>()) return launch_node_kernel<N, NODAL, SF>(p, stream);
+  if constexpr (use_node_kernel<DIM, N, SC, BCS>()) return launch_node_kernel<DIM, N, NODAL, SF>(p, stream);
   constexpr int EPW = elems_per_block<DIM, N>();
   constexpr int BLOCK = EPW * Shape<DIM, N>::LINES;
   const int64_t nb = (p.E + EPW - 1) / EPW;

@@ -92,3 +92,46 @@ def test_tgv_n7_ssprk3_steps(gpu, geometry):
     t1, t2 = gs.empty(), gs.empty()
     gs.ssprk3_steps(u, dt, 100, t1, t2)
     assert rel_l2(gs.to_reference_layout(u), ref) <= STEPS_TOL
+
+
+def vortex_2d(oc, gamma=1.4):
+    """A Mach-0.1 Taylor-Green vortex in 2D: the same cancellation as the 3D
+    state (pressure terms some 700 times the RHS)."""
+    x = oc.coordinates()
+    rho = np.ones(x.shape[:2])
+    u = np.sin(x[..., 0]) * np.cos(x[..., 1])
+    v = -np.cos(x[..., 0]) * np.sin(x[..., 1])
+    p = 100.0 / gamma + 0.25 * (np.cos(2 * x[..., 0]) + np.cos(2 * x[..., 1]))
+    return np.stack([rho, rho * u, rho * v, p / (gamma - 1) + 0.5 * rho * (u * u + v * v)], axis=-1)
+
+
+@pytest.mark.parametrize("geometry", [dg.GEOMETRY_NODAL, dg.GEOMETRY_AFFINE])
+@pytest.mark.parametrize("surface", [dg.FLUX_LLF, dg.FLUX_HLLC])
+@pytest.mark.parametrize("cells", [(3, 2), (4, 4)])
+def test_vortex_2d_n7_rhs(gpu, cells, surface, geometry):
+    print(f"case 2d vortex N=7 cells={cells} surface={surface} geometry={geometry}", flush=True)
+    oc, gs = make_pair(2, 7, cells, [0, 0], [TWO_PI] * 2, geometry=geometry, surface=surface)
+    u = vortex_2d(oc)
+    assert rel_l2(gpu_rhs(gs, u), oc.rhs(u)) <= RHS_TOL
+
+
+def test_vortex_2d_n7_ssprk3_steps(gpu):
+    print("case 2d vortex N=7 100 steps", flush=True)
+    oc, gs = make_pair(2, 7, (3, 2), [0, 0], [TWO_PI] * 2)
+    u0 = vortex_2d(oc)
+    ref = oc.ssprk3(u0, 2e-4, 100)
+    u = gs.from_reference_layout(u0)
+    t1, t2 = gs.empty(), gs.empty()
+    gs.ssprk3_steps(u, 2e-4, 100, t1, t2)
+    assert rel_l2(gs.to_reference_layout(u), ref) <= STEPS_TOL
+
+
+def test_navier_stokes_2d_n7_rhs(gpu):
+    print("case 2d ns N=7", flush=True)
+    cells, L, R = (3, 2), [0, 0], [TWO_PI] * 2
+    oc = O.Case(2, 7, cells, L, R, eq=O.NAVIER_STOKES, mu=0.05, prandtl=0.72)
+    gs = dg.StructuredSolver(dg.SolverConfig(dim=2, polydeg=7, cells=cells, domain_left=L,
+                                             domain_right=R, equation=dg.NAVIER_STOKES, mu=0.05,
+                                             prandtl=0.72))
+    u = vortex_2d(oc)
+    assert rel_l2(gpu_rhs(gs, u), oc.rhs(u)) <= NS_N7_TOL
