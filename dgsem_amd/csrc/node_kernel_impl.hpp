@@ -7,7 +7,8 @@
 // rows, which takes the kernel past 400 registers and down to one wave per
 // SIMD.
 // Here one workgroup owns one element and one thread owns one node: it
-// walks the three lines through its node and sums D_split(x, m) f(u_x, u_m)
+// walks the lines through its node (three in 3D, two in 2D) and sums
+// D_split(x, m) f(u_x, u_m)
 // over the partners m, evaluating every pair flux itself.  Each pair is thus
 // evaluated twice, once from either end (the flux is symmetric in its two
 // nodes to the bit, so both ends agree and conservation is exact), but a
