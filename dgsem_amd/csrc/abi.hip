@@ -198,6 +198,13 @@ bool has_boundaries(const dgsem_config& c) {
   return false;
 }
 
+// Navier-Stokes viscous buffer: the node-parallel RHS kernel takes the
+// per-node fluxes of every direction, the line kernels the viscous RHS part.
+int64_t visc_width(const dgsem_config& c) {
+  const bool sc = c.volume_integral == DGSEM_VOLUME_SHOCK_CAPTURING;
+  return dgsem::node_kernel_for(c.polydeg, sc, has_boundaries(c)) ? c.dim : 1;
+}
+
 int check_config(dgsem_handle* h, const dgsem_config& c) {
   if (c.dim != 2 && c.dim != 3) return fail(h, DGSEM_E_INVALID, "dim must be 2 or 3");
   if (c.polydeg < 1 || c.polydeg > 7) return fail(h, DGSEM_E_INVALID, "polydeg must be 1..7");
@@ -251,7 +258,7 @@ int launch(dgsem_handle* h, dgsem::KParams p) {
   if (h->cfg.equation == DGSEM_NAVIER_STOKES) {
     // BR1 gradient and viscous part of the stage input, then the inviscid
     // kernel adds the viscous part before -1/J
-    hipError_t e = dgsem::launch_parabolic(h->cfg.dim, h->cfg.polydeg, nodal, has_boundaries(h->cfg),
+    hipError_t e = dgsem::launch_parabolic(h->cfg.dim, h->cfg.polydeg, sc, nodal, has_boundaries(h->cfg),
                                            p, stream_of(h));
     if (e != hipSuccess) return hip_fail(h, e, "parabolic kernel launch");
     p.visc = h->d_visc;
@@ -378,7 +385,8 @@ int dgsem_create(const dgsem_config* cfg, void* stream, dgsem_handle** out) {
       (rc = upload(h, &h->d_alpha, std::vector<double>(h->E, 0.0))) ||
       (h->cfg.equation == DGSEM_NAVIER_STOKES &&
        ((rc = upload(h, &h->d_grad, std::vector<double>(h->E * h->cfg.dim * h->nvars * h->nn, 0.0))) ||
-        (rc = upload(h, &h->d_visc, std::vector<double>(h->E * h->nvars * h->nn, 0.0)))))) {
+        (rc = upload(h, &h->d_visc, std::vector<double>(h->E * visc_width(h->cfg) * h->nvars * h->nn,
+                                                        0.0)))))) {
     release(h);
     *out = h;
     return rc;

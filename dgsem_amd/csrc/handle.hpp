@@ -26,7 +26,8 @@ struct dgsem_handle {
   double* d_Vinv = nullptr;
   double* d_alpha = nullptr;
   double* d_grad = nullptr;  // Navier-Stokes gradient [E][dim][var][dof]
-  double* d_visc = nullptr;  // Navier-Stokes viscous RHS part [E][var][dof]
+  double* d_visc = nullptr;  // Navier-Stokes viscous RHS part [E][var][dof] or
+                             // per-node fluxes [E][dim][var][dof] (node_kernel_for)
   dgsem::KParams base{};
   // tangent / adjoint scratch, allocated on first use: d_ad_dgrad
   // [E][dim][NV][NN], d_ad_dvisc and d_ad_qbar [E][NV][NN] (Navier-Stokes),

@@ -10,6 +10,18 @@ namespace dgsem {
 
 constexpr int kMaxN1 = 8;  // max nodes per direction (N <= 7)
 
+#ifndef DGSEM_NODE_KERNEL_MIN_N
+#define DGSEM_NODE_KERNEL_MIN_N 7
+#endif
+
+// Split-form elements of degree N >= 7 on periodic meshes run the
+// node-parallel kernel, in 2D and 3D.  With Navier-Stokes terms that kernel
+// takes the viscous fluxes Ja^d . (-F_v) per node [E][dim][var][dof]
+// (instead of their divergence [E][var][dof]) and adds their terms itself.
+__host__ __device__ constexpr bool node_kernel_for(int N, bool shock_capturing, bool boundaries) {
+  return !shock_capturing && !boundaries && N >= DGSEM_NODE_KERNEL_MIN_N;
+}
+
 enum StageMode : int {
   kModeRhs = 0,     // out = L(u)
   kModeStage2 = 1,  // out = c1 u + c2 L(u)                (parallel_ma2)
@@ -68,12 +80,15 @@ template <int DIM, int N>
 hipError_t launch_rhs_dn(bool shock_capturing, bool nodal, bool boundaries, const KParams& p,
                          hipStream_t stream);
 template <int DIM, int N>
-hipError_t launch_parabolic_dn(bool nodal, bool boundaries, const KParams& p, hipStream_t stream);
+hipError_t launch_parabolic_dn(bool shock_capturing, bool nodal, bool boundaries, const KParams& p,
+                               hipStream_t stream);
 
 // Gradient and viscous kernels of the Navier-Stokes terms: fill p.grad_out
-// (with p.grad == p.grad_out) and p.visc_out from p.u.
-hipError_t launch_parabolic(int dim, int N, bool nodal, bool boundaries, const KParams& p,
-                            hipStream_t stream);
+// (with p.grad == p.grad_out) and p.visc_out from p.u; p.visc_out takes the
+// per-node viscous fluxes when node_kernel_for(N, shock_capturing,
+// boundaries), else the viscous part of the RHS.
+hipError_t launch_parabolic(int dim, int N, bool shock_capturing, bool nodal, bool boundaries,
+                            const KParams& p, hipStream_t stream);
 
 // Launches the fused RHS / stage kernel for the configuration; returns
 // hipErrorInvalidValue when (dim, N, shock capturing, nodal) is not built.

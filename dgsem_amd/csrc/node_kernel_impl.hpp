@@ -40,15 +40,9 @@
 namespace dgsem {
 namespace detail {
 
-#ifndef DGSEM_NODE_KERNEL_MIN_N
-#define DGSEM_NODE_KERNEL_MIN_N 7
-#endif
-
-// Split-form elements of degree N >= 7 on periodic meshes run the
-// node-parallel kernel, in 2D and 3D.
 template <int DIM, int N, bool SC, bool BCS>
 constexpr bool use_node_kernel() {
-  return !SC && !BCS && N >= DGSEM_NODE_KERNEL_MIN_N;
+  return node_kernel_for(N, SC, BCS);
 }
 
 // The functions below round every operation as the reference's scalar code
@@ -280,6 +274,7 @@ __global__ void __launch_bounds__((Shape<DIM, N>::NN)) __attribute__((amdgpu_wav
   __shared__ double s_q[(DIM + 3) * PN];             // rho, v[DIM], beta, kin
   __shared__ double s_ja[NODAL ? DIM * DIM * PN : 1];  // Ja^d_c / 2
   __shared__ double s_D[n * DS];
+  __shared__ double s_Dh[n * DS];  // D_hat (Navier-Stokes viscous terms)
 
   const int a = threadIdx.x;
   const int64_t e = xcd_block(blockIdx.x, gridDim.x);
@@ -314,7 +309,10 @@ __global__ void __launch_bounds__((Shape<DIM, N>::NN)) __attribute__((amdgpu_wav
 #pragma unroll
       for (int c = 0; c < DIM; ++c) s_ja[(d * DIM + c) * PN + pa] = 0.5 * jm[d][c];
   }
-  if (a < n * n) s_D[(a / n) * DS + a % n] = p.Ds[a / n][a % n];
+  if (a < n * n) {
+    s_D[(a / n) * DS + a % n] = p.Ds[a / n][a % n];
+    s_Dh[(a / n) * DS + a % n] = p.Dhat[a / n][a % n];
+  }
   __syncthreads();
 
   // 3. flux differencing.  The reference (volume_flux.hpp:296-410) visits
@@ -383,6 +381,71 @@ __global__ void __launch_bounds__((Shape<DIM, N>::NN)) __attribute__((amdgpu_wav
 #pragma unroll 1
     for (int m = iz + 1; m <= N; ++m) term(2, iz, n * n, m, jz);
   }
+  // 3b. Navier-Stokes: the viscous volume integral adds
+  // D_hat(x_d, m) Ja^d . (-F_v)(m) over every partner m of each line through
+  // the node (the node itself included) into the same sums, after the
+  // split-form terms (parabolic_viscous.hpp:103-163 3D, :412-447 2D).  It
+  // visits the source nodes in memory order, so a node takes: sources below
+  // it on its z-line and y-line, its x-line up to itself, its own y and z
+  // terms, then the rest of its x-, y- and z-lines.  At a face node the sums
+  // hold the uncancelled half of the face balance, about a thousand times
+  // the RHS, so this order fixes the RHS's last bits whatever the viscosity.
+  const double* vf = p.visc;  // [E][d][var][dof], or null (Euler)
+  auto vterm = [&](int d, int xd, int sd, int m) {
+    const int b = a + (m - xd) * sd;
+    const double w = s_Dh[xd * DS + m];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+#pragma clang fp contract(off)
+      acc[v] = acc[v] + w * vf[((e * DIM + d) * NV + v) * NN + b];
+    }
+  };
+  if (vf) {
+    if constexpr (DIM == 3) {
+#pragma unroll 1
+      for (int m = 0; m < iz; ++m) vterm(2, iz, n * n, m);
+    }
+#pragma unroll 1
+    for (int m = 0; m < iy; ++m) vterm(1, iy, n, m);
+#pragma unroll 1
+    for (int m = 0; m <= ix; ++m) vterm(0, ix, 1, m);
+    vterm(1, iy, n, iy);
+    if constexpr (DIM == 3) vterm(2, iz, n * n, iz);
+#pragma unroll 1
+    for (int m = ix + 1; m <= N; ++m) vterm(0, ix, 1, m);
+#pragma unroll 1
+    for (int m = iy + 1; m <= N; ++m) vterm(1, iy, n, m);
+    if constexpr (DIM == 3) {
+#pragma unroll 1
+      for (int m = iz + 1; m <= N; ++m) vterm(2, iz, n * n, m);
+    }
+  }
+  const double own_sg = ij >= 0.0 ? 1.0 : -1.0;
+  // 3c. the viscous interface terms in direction order
+  // (structured_solver.hpp:359-360 with the one-sided flux of
+  // parabolic_viscous.hpp:318-334 / :570-585): the lower face takes the
+  // node's own (-F_v) . (sign(J) Ja^d), the upper face the upper
+  // neighbour's at its node with index 0.
+  if (vf) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const bool lower = idx[d] == 0, upper = idx[d] == N;
+      if (lower || upper) {
+        int64_t lo, hi;
+        face_neighbours<DIM, false>(p, p.cells[0], p.cells[1], nz, e, d, lo, hi);
+        const int64_t ve = lower ? e : hi;
+        const int vn = lower ? a : a - N * stride[d];
+        const double vij = lower ? ij : (NODAL ? p.invJ[ve * NN + vn] : p.invJ[ve]);
+        const double sg = vij >= 0.0 ? 1.0 : -1.0;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+#pragma clang fp contract(off)
+          const double fp = sg * vf[((ve * DIM + d) * NV + v) * NN + vn];
+          acc[v] = lower ? acc[v] - fp * p.lhat0 : acc[v] + fp * p.lhatN;
+        }
+      }
+    }
+  }
   // 4. numerical fluxes on the faces this node lies on, lifted after the
   // volume terms in direction order.  Their inputs are loaded here, not at
   // the top, so that nothing but the sums lives through the term loops (a
@@ -390,7 +453,6 @@ __global__ void __launch_bounds__((Shape<DIM, N>::NN)) __attribute__((amdgpu_wav
   // neighbour's node with index N, the upper face the upper neighbour's node
   // with index 0 and its metric row and 1/J, whose sign and row make the
   // interface normal; interior nodes load nothing.
-  const double own_sg = ij >= 0.0 ? 1.0 : -1.0;
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
     const bool lower = idx[d] == 0, upper = idx[d] == N;
@@ -433,7 +495,7 @@ __global__ void __launch_bounds__((Shape<DIM, N>::NN)) __attribute__((amdgpu_wav
   }
   const double* tot = acc;
 
-  // 5. the viscous part, -1/J and the stage combination.  1/J, the base
+  // 5. -1/J and the stage combination.  1/J, the base
   // state and the result are touched once per launch: non-temporal accesses
   // keep them from displacing the neighbours' face traces in the L2.
   const int64_t g = e * NV * NN + a;
@@ -450,16 +512,8 @@ __global__ void __launch_bounds__((Shape<DIM, N>::NN)) __attribute__((amdgpu_wav
 #pragma unroll
   for (int v = 0; v < NV; ++v) uk[v] = p.u[g + v * NN];
   double r[NV];
-  if (p.visc) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-#pragma clang fp contract(off)
-      r[v] = (tot[v] + p.visc[g + v * NN]) * fac;
-    }
-  } else {
-#pragma unroll
-    for (int v = 0; v < NV; ++v) r[v] = tot[v] * fac;
-  }
+  for (int v = 0; v < NV; ++v) r[v] = tot[v] * fac;
   if (p.mode == kModeRhs) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) __builtin_nontemporal_store(r[v], &p.out[g + v * NN]);

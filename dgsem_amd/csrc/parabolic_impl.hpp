@@ -14,7 +14,11 @@
 //     flux, which is the upper ("primary") element's (-F_v).n only
 //     (parabolic_viscous.hpp:318-334 3D, :570-585 2D), lifted with l_hat.
 //     The result (before -1/J) is added by the inviscid kernel.
-// Layouts: gradient [E][dim][var][dof]; viscous part [E][var][dof].
+//     Elements of the node-parallel RHS kernel (node_kernel_for) take
+//     viscous_node_flux_kernel instead, which stores the per-node fluxes
+//     only; that kernel adds their terms in the reference's order.
+// Layouts: gradient [E][dim][var][dof]; viscous part [E][var][dof], or the
+// per-node fluxes [E][dim][var][dof].
 // BCS (2D): some face is not periodic; the neighbour table holds -1 there and
 // the face values come from the boundary condition (wall_gradient_value and
 // the viscous kernel's wall branch).
@@ -32,6 +36,7 @@ namespace detail {
 // gradient_variables (rho, v, T = p/rho), compressible_navier_stokes3D.hpp:55-70
 template <int DIM>
 __device__ __forceinline__ void grad_vars(double gamma, const double* u, double* q) {
+#pragma clang fp contract(off)
   const double inv_rho = 1.0 / u[0];
   double m2 = 0.0;
   for (int k = 0; k < DIM; ++k) m2 += u[k + 1] * u[k + 1];
@@ -46,6 +51,7 @@ __device__ __forceinline__ void grad_vars(double gamma, const double* u, double*
 template <int DIM>
 __device__ __forceinline__ void viscous_flux_dir(double mu, double kappa, const double* q,
                                                  const double g[DIM][DIM + 2], int dim, double* f) {
+#pragma clang fp contract(off)
   if constexpr (DIM == 2) {
     const double tau11 = (4.0 * g[0][1] - 2.0 * g[1][2]) / 3.0;
     const double tau22 = (4.0 * g[1][2] - 2.0 * g[0][1]) / 3.0;
@@ -361,6 +367,137 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) viscous_kernel(c
   }
 }
 
+// BR1 gradient for the node-parallel RHS kernel's elements (periodic, no
+// shock capturing, N >= 7), one workgroup per element and one thread per
+// node, rounded as the reference's scalar functors round it: the weak
+// derivative sum_m D_hat(x_d, m) q(m) of each direction d from left to right
+// (GradientVolumeIntegralFunctor, parabolic_gradient.hpp:40-79 3D,
+// :502-527 2D), the physical transform Ja^0_c r_0 + Ja^1_c r_1 (+ Ja^2_c r_2)
+// (:121-150 / :567-584), the lifted face values (-l_0 or l_N) q_hat times
+// sign(J) Ja^d_c, q_hat = (q_lower + q_upper) / 2, in direction order
+// (:186-291 and :348-430 / :628-685 and :737-788), then 1/J (:462-474 /
+// :806-815); no fused multiply-adds.  The derivative of T = p / rho cancels
+// terms some thousand times its size at N = 7, so these roundings show in
+// the viscous terms at the 1e-12 level.
+template <int DIM, int N, bool NODAL>
+__global__ void __launch_bounds__((Shape<DIM, N>::NN)) gradient_node_kernel(const KParams p) {
+  constexpr int n = N + 1, NV = DIM + 2, NN = Shape<DIM, N>::NN;
+  __shared__ double s_q[NV * NN];
+  __shared__ double s_Dh[n * n];
+  const int a = threadIdx.x;
+  const int64_t e = xcd_block(blockIdx.x, gridDim.x);
+  const int idx[3] = {a % n, (a / n) % n, DIM == 3 ? a / (n * n) : 0};
+  constexpr int stride[3] = {1, n, n * n};
+  const int nz = DIM == 3 ? p.cells[2] : 1;
+  double u[NV], q[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) u[v] = p.u[(e * NV + v) * NN + a];
+  grad_vars<DIM>(p.gamma, u, q);
+#pragma unroll
+  for (int v = 0; v < NV; ++v) s_q[v * NN + a] = q[v];
+  if (a < n * n) s_Dh[a] = p.Dhat[a / n][a % n];
+  __syncthreads();
+  double r[DIM][NV];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      double t = 0.0;
+#pragma unroll
+      for (int m = 0; m < n; ++m) {
+#pragma clang fp contract(off)
+        t = t + s_Dh[idx[d] * n + m] * s_q[v * NN + a + (m - idx[d]) * stride[d]];
+      }
+      r[d][v] = t;
+    }
+  double g[DIM][NV];
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) {
+    double ja[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) ja[d] = metric<NODAL, DIM, NN>(p, e, d, c, a);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+#pragma clang fp contract(off)
+      double t = ja[0] * r[0][v];
+#pragma unroll
+      for (int d = 1; d < DIM; ++d) t = t + ja[d] * r[d][v];
+      g[c][v] = t;
+    }
+  }
+  const double ij = inv_jac<NODAL, NN>(p, e, a);
+  const double sg = ij >= 0.0 ? 1.0 : -1.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const bool lower = idx[d] == 0, upper = idx[d] == N;
+    if (lower || upper) {
+      int64_t lo, hi;
+      face_neighbours<DIM, false>(p, p.cells[0], p.cells[1], nz, e, d, lo, hi);
+      const int64_t ne = lower ? lo : hi;
+      const int nn = lower ? a + N * stride[d] : a - N * stride[d];
+      double un[NV], qn[NV], nrm[DIM];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) un[v] = p.u[(ne * NV + v) * NN + nn];
+      grad_vars<DIM>(p.gamma, un, qn);
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) nrm[c] = sg * metric<NODAL, DIM, NN>(p, e, d, c, a);
+      const double factor = lower ? -p.lhat0 : p.lhatN;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+#pragma clang fp contract(off)
+        const double fq = factor * (0.5 * (qn[v] + q[v]));
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) g[c][v] = g[c][v] + fq * nrm[c];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < DIM; ++c)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) p.grad_out[((e * DIM + c) * NV + v) * NN + a] = g[c][v] * ij;
+}
+
+// Viscous fluxes for the node-parallel RHS kernel: at every node the
+// contravariant flux Ja^d . (-F_v) of each direction d, out [E][d][var][dof],
+// rounded as ViscousFluxFunctor and the viscous volume integral's fref
+// (parabolic_viscous.hpp:39-66 and :103-163 3D, :367-386 and :412-447 2D):
+// -F_v of each physical direction, then Ja^d_0 f_0 + Ja^d_1 f_1 (+ Ja^d_2 f_2)
+// from left to right, with no fused multiply-adds.  The interface value
+// (-F_v) . (sign(J) Ja^d) of a face node is sign(J) times this sum to the
+// bit (scaling by a sign is exact), so the RHS kernel needs nothing more.
+template <int DIM, int N, bool NODAL>
+__global__ void __launch_bounds__(256) viscous_node_flux_kernel(const KParams p) {
+  constexpr int NV = DIM + 2, NN = Shape<DIM, N>::NN;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= p.E * NN) return;
+  const int64_t e = i / NN;
+  const int node = (int)(i % NN);
+  double s[NV], q[NV], g[DIM][NV], fv[DIM][NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) s[v] = p.u[(e * NV + v) * NN + node];
+  grad_vars<DIM>(p.gamma, s, q);
+#pragma unroll
+  for (int c = 0; c < DIM; ++c)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) g[c][v] = p.grad[((e * DIM + c) * NV + v) * NN + node];
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) viscous_flux_dir<DIM>(p.mu, p.kappa, q, g, c, fv[c]);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    double ja[DIM];
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) ja[c] = metric<NODAL, DIM, NN>(p, e, d, c, node);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+#pragma clang fp contract(off)
+      double r = ja[0] * -fv[0][v];
+#pragma unroll
+      for (int c = 1; c < DIM; ++c) r = r + ja[c] * -fv[c][v];
+      p.visc_out[((e * DIM + d) * NV + v) * NN + node] = r;
+    }
+  }
+}
+
 template <int DIM, int N>
 constexpr int parabolic_elems_per_block() {
   constexpr int e = 192 / Shape<DIM, N>::LINES;
@@ -368,11 +505,23 @@ constexpr int parabolic_elems_per_block() {
 }
 
 template <int DIM, int N, bool NODAL, bool BCS>
-hipError_t launch_parabolic_t(const KParams& p, hipStream_t s) {
+hipError_t launch_parabolic_t(bool sc, const KParams& p, hipStream_t s) {
   constexpr int EPW = parabolic_elems_per_block<DIM, N>();
   constexpr int BLOCK = EPW * Shape<DIM, N>::LINES;
   const int64_t nb = (p.E + EPW - 1) / EPW;
   if (nb <= 0) return hipSuccess;
+  if constexpr (node_kernel_for(N, false, BCS)) {
+    if (!sc) {
+      hipLaunchKernelGGL((gradient_node_kernel<DIM, N, NODAL>), dim3((unsigned)p.E),
+                         dim3(Shape<DIM, N>::NN), 0, s, p);
+      hipError_t err = hipGetLastError();
+      if (err != hipSuccess) return err;
+      const int64_t nodes = p.E * Shape<DIM, N>::NN;
+      hipLaunchKernelGGL((viscous_node_flux_kernel<DIM, N, NODAL>),
+                         dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, s, p);
+      return hipGetLastError();
+    }
+  }
   hipLaunchKernelGGL((gradient_kernel<DIM, N, EPW, NODAL, BCS>), dim3((unsigned)nb), dim3(BLOCK), 0, s, p);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;

@@ -80,25 +80,26 @@ hipError_t launch_rhs(int dim, int N, bool sc, bool nodal, bool bcs, const KPara
   }
   return hipErrorInvalidValue;
 }
-hipError_t launch_parabolic(int dim, int N, bool nodal, bool bcs, const KParams& p, hipStream_t s) {
+hipError_t launch_parabolic(int dim, int N, bool sc, bool nodal, bool bcs, const KParams& p,
+                            hipStream_t s) {
   if (dim == 2) {
     switch (N) {
-      case 1: return launch_parabolic_dn<2, 1>(nodal, bcs, p, s);
-      case 2: return launch_parabolic_dn<2, 2>(nodal, bcs, p, s);
-      case 3: return launch_parabolic_dn<2, 3>(nodal, bcs, p, s);
-      case 4: return launch_parabolic_dn<2, 4>(nodal, bcs, p, s);
-      case 5: return launch_parabolic_dn<2, 5>(nodal, bcs, p, s);
-      case 7: return launch_parabolic_dn<2, 7>(nodal, bcs, p, s);
+      case 1: return launch_parabolic_dn<2, 1>(sc, nodal, bcs, p, s);
+      case 2: return launch_parabolic_dn<2, 2>(sc, nodal, bcs, p, s);
+      case 3: return launch_parabolic_dn<2, 3>(sc, nodal, bcs, p, s);
+      case 4: return launch_parabolic_dn<2, 4>(sc, nodal, bcs, p, s);
+      case 5: return launch_parabolic_dn<2, 5>(sc, nodal, bcs, p, s);
+      case 7: return launch_parabolic_dn<2, 7>(sc, nodal, bcs, p, s);
       default: return hipErrorInvalidValue;
     }
   }
   if (dim == 3) {
     switch (N) {
-      case 2: return launch_parabolic_dn<3, 2>(nodal, bcs, p, s);
-      case 3: return launch_parabolic_dn<3, 3>(nodal, bcs, p, s);
-      case 4: return launch_parabolic_dn<3, 4>(nodal, bcs, p, s);
-      case 5: return launch_parabolic_dn<3, 5>(nodal, bcs, p, s);
-      case 7: return launch_parabolic_dn<3, 7>(nodal, bcs, p, s);
+      case 2: return launch_parabolic_dn<3, 2>(sc, nodal, bcs, p, s);
+      case 3: return launch_parabolic_dn<3, 3>(sc, nodal, bcs, p, s);
+      case 4: return launch_parabolic_dn<3, 4>(sc, nodal, bcs, p, s);
+      case 5: return launch_parabolic_dn<3, 5>(sc, nodal, bcs, p, s);
+      case 7: return launch_parabolic_dn<3, 7>(sc, nodal, bcs, p, s);
       default: return hipErrorInvalidValue;
     }
   }

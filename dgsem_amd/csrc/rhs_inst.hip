@@ -15,16 +15,16 @@ hipError_t launch_rhs_dn<DGSEM_INST_DIM, DGSEM_INST_N>(bool sc, bool nodal, bool
   return detail::launch_dn<DGSEM_INST_DIM, DGSEM_INST_N>(sc, nodal, bcs, p, s);
 }
 template <>
-hipError_t launch_parabolic_dn<DGSEM_INST_DIM, DGSEM_INST_N>(bool nodal, bool bcs, const KParams& p,
-                                                              hipStream_t s) {
+hipError_t launch_parabolic_dn<DGSEM_INST_DIM, DGSEM_INST_N>(bool sc, bool nodal, bool bcs,
+                                                              const KParams& p, hipStream_t s) {
 #if DGSEM_INST_DIM == 2
   if (bcs)
-    return nodal ? detail::launch_parabolic_t<DGSEM_INST_DIM, DGSEM_INST_N, true, true>(p, s)
-                 : detail::launch_parabolic_t<DGSEM_INST_DIM, DGSEM_INST_N, false, true>(p, s);
+    return nodal ? detail::launch_parabolic_t<DGSEM_INST_DIM, DGSEM_INST_N, true, true>(sc, p, s)
+                 : detail::launch_parabolic_t<DGSEM_INST_DIM, DGSEM_INST_N, false, true>(sc, p, s);
 #else
   if (bcs) return hipErrorInvalidValue;  // non-periodic boundaries are built for 2D
 #endif
-  return nodal ? detail::launch_parabolic_t<DGSEM_INST_DIM, DGSEM_INST_N, true, false>(p, s)
-               : detail::launch_parabolic_t<DGSEM_INST_DIM, DGSEM_INST_N, false, false>(p, s);
+  return nodal ? detail::launch_parabolic_t<DGSEM_INST_DIM, DGSEM_INST_N, true, false>(sc, p, s)
+               : detail::launch_parabolic_t<DGSEM_INST_DIM, DGSEM_INST_N, false, false>(sc, p, s);
 }
 }  // namespace dgsem

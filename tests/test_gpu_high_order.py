@@ -1,12 +1,12 @@
-"""GPU parity of high-degree 3D split-form elements, which run the
-node-parallel kernel (one thread per node, every pair flux evaluated from
+"""GPU parity of high-degree split-form elements, which run the
+node-parallel kernels (one thread per node, every pair flux evaluated from
 both ends): Taylor-Green and strongly varying states (the logarithmic means'
 log branch), both surface fluxes, both geometry modes, Navier-Stokes, and
 SSPRK3 steps, against the oracle.  Bars as everywhere: one RHS relative L2
-<= 1e-12, steps <= 1e-10, plus conservation of the global sums; the
-Navier-Stokes case has its own bar (below).  The 3x2x4 TGV at N = 7 is
-the hardest case: its RHS is some 700 times smaller than the pressure
-terms its sums cancel."""
+<= 1e-12, steps <= 1e-10, plus conservation of the global sums.  The 3x2x4
+TGV at N = 7 is the hardest case: its RHS is some 700 times smaller than
+the pressure terms its sums cancel, and with viscosity the derivative of
+the temperature cancels terms some thousand times its size."""
 import math
 
 import numpy as np
@@ -60,25 +60,24 @@ def test_rough_state_rhs_and_conservation(gpu, N, cells):
     assert np.all(np.abs(tot) <= 1e-14 * scale)
 
 
-# The BR1 gradient and viscous kernels keep their own summation order and
-# fused roundings.  At N = 7 the split-form sums alone move the RHS by 1.3e-12
-# relative when the oracle's own terms are summed per direction instead of
-# in its order (measured on the 3x2x4 TGV), so these kernels' terms are held
-# to a wider bar than the inviscid kernel, which follows the reference's
-# order and roundings.
-NS_N7_TOL = 5e-12
-
-
+@pytest.mark.parametrize("geometry", [dg.GEOMETRY_NODAL, dg.GEOMETRY_AFFINE])
 @pytest.mark.parametrize("mu", [6.25e-4, 0.05])
-def test_navier_stokes_n7_rhs(gpu, mu):
-    print(f"case ns N=7 mu={mu}", flush=True)
-    cells, L, R = (2, 3, 2), [0, 0, 0], [TWO_PI] * 3
-    oc = O.Case(3, 7, cells, L, R, eq=O.NAVIER_STOKES, mu=mu, prandtl=0.72)
-    gs = dg.StructuredSolver(dg.SolverConfig(dim=3, polydeg=7, cells=cells, domain_left=L,
-                                             domain_right=R, equation=dg.NAVIER_STOKES, mu=mu,
-                                             prandtl=0.72))
+def test_navier_stokes_n7_rhs(gpu, mu, geometry):
+    print(f"case ns N=7 mu={mu} geometry={geometry}", flush=True)
+    oc, gs = make_pair(3, 7, (2, 3, 2), [0, 0, 0], [TWO_PI] * 3, geometry=geometry, mu=mu)
     u = oc.init_tgv()
-    assert rel_l2(gpu_rhs(gs, u), oc.rhs(u)) <= NS_N7_TOL
+    assert rel_l2(gpu_rhs(gs, u), oc.rhs(u)) <= RHS_TOL
+
+
+def test_navier_stokes_n7_ssprk3_steps(gpu):
+    print("case ns N=7 mu=0.05 100 steps", flush=True)
+    oc, gs = make_pair(3, 7, (2, 2, 2), [0, 0, 0], [TWO_PI] * 3, mu=0.05)
+    u0 = oc.init_tgv()
+    ref = oc.ssprk3(u0, 2e-4, 100)
+    u = gs.from_reference_layout(u0)
+    t1, t2 = gs.empty(), gs.empty()
+    gs.ssprk3_steps(u, 2e-4, 100, t1, t2)
+    assert rel_l2(gs.to_reference_layout(u), ref) <= STEPS_TOL
 
 
 @pytest.mark.parametrize("geometry", [dg.GEOMETRY_NODAL, dg.GEOMETRY_AFFINE])
@@ -126,15 +125,23 @@ def test_vortex_2d_n7_ssprk3_steps(gpu):
     assert rel_l2(gs.to_reference_layout(u), ref) <= STEPS_TOL
 
 
-def test_navier_stokes_2d_n7_rhs(gpu):
-    print("case 2d ns N=7", flush=True)
-    cells, L, R = (3, 2), [0, 0], [TWO_PI] * 2
-    oc = O.Case(2, 7, cells, L, R, eq=O.NAVIER_STOKES, mu=0.05, prandtl=0.72)
-    gs = dg.StructuredSolver(dg.SolverConfig(dim=2, polydeg=7, cells=cells, domain_left=L,
-                                             domain_right=R, equation=dg.NAVIER_STOKES, mu=0.05,
-                                             prandtl=0.72))
+@pytest.mark.parametrize("geometry", [dg.GEOMETRY_NODAL, dg.GEOMETRY_AFFINE])
+def test_navier_stokes_2d_n7_rhs(gpu, geometry):
+    print(f"case 2d ns N=7 geometry={geometry}", flush=True)
+    oc, gs = make_pair(2, 7, (3, 2), [0, 0], [TWO_PI] * 2, geometry=geometry, mu=0.05)
     u = vortex_2d(oc)
-    assert rel_l2(gpu_rhs(gs, u), oc.rhs(u)) <= NS_N7_TOL
+    assert rel_l2(gpu_rhs(gs, u), oc.rhs(u)) <= RHS_TOL
+
+
+def test_navier_stokes_2d_n7_ssprk3_steps(gpu):
+    print("case 2d ns N=7 100 steps", flush=True)
+    oc, gs = make_pair(2, 7, (3, 2), [0, 0], [TWO_PI] * 2, mu=0.05)
+    u0 = vortex_2d(oc)
+    ref = oc.ssprk3(u0, 2e-4, 100)
+    u = gs.from_reference_layout(u0)
+    t1, t2 = gs.empty(), gs.empty()
+    gs.ssprk3_steps(u, 2e-4, 100, t1, t2)
+    assert rel_l2(gs.to_reference_layout(u), ref) <= STEPS_TOL
 
 
 @pytest.mark.parametrize("N,cells,tol", [(7, 32, 1e-11), (3, 64, 1e-12)])
