@@ -47,6 +47,8 @@ CASES = [
     (3, 2, (3, 2, 4), dg.NAVIER_STOKES, dg.GEOMETRY_AFFINE),
     (3, 5, (2, 2, 2), dg.EULER, dg.GEOMETRY_AFFINE),
     (3, 7, (2, 1, 2), dg.EULER, dg.GEOMETRY_NODAL),
+    (3, 7, (2, 1, 2), dg.NAVIER_STOKES, dg.GEOMETRY_NODAL),
+    (2, 7, (3, 2), dg.NAVIER_STOKES, dg.GEOMETRY_AFFINE),
     (2, 3, (6, 5), dg.EULER, dg.GEOMETRY_NODAL),
     (2, 4, (4, 4), dg.NAVIER_STOKES, dg.GEOMETRY_AFFINE),
     (2, 1, (5, 4), dg.NAVIER_STOKES, dg.GEOMETRY_NODAL),
