@@ -299,9 +299,9 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the RHS would shard by element blocks with a face-trace halo exchange; "
-              "see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path "
+              "(the RHS would shard by element blocks with a face-trace halo exchange). "
+              "Run with --gpus 1.", file=sys.stderr)
         return 2
 
     import torch
