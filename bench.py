@@ -258,6 +258,25 @@ def tgv_high_order_measure(cells: int = 32, N: int = 7, steps: int = 10) -> dict
             "volume_tflops": tflops, "frac_fp64_peak": tflops / FP64_PEAK_TFLOPS}
 
 
+def ns_high_order_measure(cells: int = 32, N: int = 7, steps: int = 10, mu: float = 6.25e-4) -> dict:
+    """The Navier-Stokes Taylor-Green vortex (Re = 1600 at mu = 6.25e-4,
+    Pr = 0.72) at N = 7 on 32^3 elements: step rate of the BR1 path whose
+    gradient, viscous fluxes and sums follow the reference's order and
+    roundings (two node-parallel BR1 kernels and the node RHS kernel per
+    stage)."""
+    import dgsem_amd as dg
+
+    cfg = dg.SolverConfig(dim=3, polydeg=N, cells=(cells,) * 3, domain_left=(0, 0, 0),
+                          domain_right=(2 * math.pi,) * 3, equation=dg.NAVIER_STOKES, mu=mu,
+                          prandtl=0.72)
+    s = dg.StructuredSolver(cfg)
+    u = s.from_reference_layout(tgv_state(s.node_coordinates()))
+    dt = 0.5 * (2 * math.pi / cells) / (3 * (2 * N + 1) * (1.0 + math.sqrt(1.4 * (100.0 / 1.4 + 0.25))))
+    ms, rate, stage_ms = _step_rate(s, u, dt, steps, 2)
+    return {"workload": f"tgv3d_navier_stokes_{cells}^3_N{N}_mu{mu}_splitform_llf_br1_ssprk3",
+            "nodes": s.n_elements * s.ndofs, "ms_per_step": ms, "dof_updates_per_s": rate}
+
+
 def sedov_measure(cells: int = 32, N: int = 3, steps: int = 20) -> dict:
     """The 3D Sedov blast with subcell shock capturing (alpha_max 0.5, HLLC
     subcell flux, positivity floor 1e-13) on 32^3 elements at N = 3: step
@@ -293,6 +312,8 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-ensemble and adjoint measurements")
+    ap.add_argument("--ns-n7", action="store_true",
+                    help="also time the Navier-Stokes TGV at N = 7 on 32^3 elements")
     ap.add_argument("--profile-copy-mib", type=int, default=0,
                     help="after the run, launch an elementwise copy kernel of this size "
                          "(calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -389,6 +410,8 @@ def main() -> int:
         out["geometry"] = geometry_measure()
         out["tgv_n7"] = tgv_high_order_measure()
         out["sedov"] = sedov_measure()
+    if args.ns_n7:
+        out["ns_n7"] = ns_high_order_measure()
     if args.profile_copy_mib > 0:
         a = torch.ones(args.profile_copy_mib * (1 << 20) // 8, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)

@@ -15,7 +15,7 @@
 //     (parabolic_viscous.hpp:318-334 3D, :570-585 2D), lifted with l_hat.
 //     The result (before -1/J) is added by the inviscid kernel.
 //     Elements of the node-parallel RHS kernel (node_kernel_for) take
-//     viscous_node_flux_kernel instead, which stores the per-node fluxes
+//     gradient_node_kernel instead, which stores the per-node viscous fluxes
 //     only; that kernel adds their terms in the reference's order.
 // Layouts: gradient [E][dim][var][dof]; viscous part [E][var][dof], or the
 // per-node fluxes [E][dim][var][dof].
@@ -367,6 +367,38 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) viscous_kernel(c
   }
 }
 
+// Viscous fluxes for the node-parallel RHS kernel: at node `node` of
+// element e the contravariant flux Ja^d . (-F_v) of each direction d, into
+// p.visc_out [E][d][var][dof], rounded as ViscousFluxFunctor and the viscous
+// volume integral's fref (parabolic_viscous.hpp:39-66 and :103-163 3D,
+// :367-386 and :412-447 2D): -F_v of each physical direction, then
+// Ja^d_0 f_0 + Ja^d_1 f_1 (+ Ja^d_2 f_2) from left to right, with no fused
+// multiply-adds.  The interface value (-F_v) . (sign(J) Ja^d) of a face node
+// is sign(J) times this sum to the bit (scaling by a sign is exact), so the
+// RHS kernel needs nothing more.
+template <int DIM, int N, bool NODAL>
+__device__ __forceinline__ void store_viscous_fluxes(const KParams& p, int64_t e, int node,
+                                                     const double* q, const double g[DIM][DIM + 2]) {
+  constexpr int NV = DIM + 2, NN = Shape<DIM, N>::NN;
+  double fv[DIM][NV];
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) viscous_flux_dir<DIM>(p.mu, p.kappa, q, g, c, fv[c]);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    double ja[DIM];
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) ja[c] = metric<NODAL, DIM, NN>(p, e, d, c, node);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+#pragma clang fp contract(off)
+      double r = ja[0] * -fv[0][v];
+#pragma unroll
+      for (int c = 1; c < DIM; ++c) r = r + ja[c] * -fv[c][v];
+      p.visc_out[((e * DIM + d) * NV + v) * NN + node] = r;
+    }
+  }
+}
+
 // BR1 gradient for the node-parallel RHS kernel's elements (periodic, no
 // shock capturing, N >= 7), one workgroup per element and one thread per
 // node, rounded as the reference's scalar functors round it: the weak
@@ -378,7 +410,8 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) viscous_kernel(c
 // (:186-291 and :348-430 / :628-685 and :737-788), then 1/J (:462-474 /
 // :806-815); no fused multiply-adds.  The derivative of T = p / rho cancels
 // terms some thousand times its size at N = 7, so these roundings show in
-// the viscous terms at the 1e-12 level.
+// the viscous terms at the 1e-12 level.  Writes the node's viscous fluxes
+// (store_viscous_fluxes), not the gradient.
 template <int DIM, int N, bool NODAL>
 __global__ void __launch_bounds__((Shape<DIM, N>::NN)) gradient_node_kernel(const KParams p) {
   constexpr int n = N + 1, NV = DIM + 2, NN = Shape<DIM, N>::NN;
@@ -454,48 +487,10 @@ __global__ void __launch_bounds__((Shape<DIM, N>::NN)) gradient_node_kernel(cons
 #pragma unroll
   for (int c = 0; c < DIM; ++c)
 #pragma unroll
-    for (int v = 0; v < NV; ++v) p.grad_out[((e * DIM + c) * NV + v) * NN + a] = g[c][v] * ij;
-}
-
-// Viscous fluxes for the node-parallel RHS kernel: at every node the
-// contravariant flux Ja^d . (-F_v) of each direction d, out [E][d][var][dof],
-// rounded as ViscousFluxFunctor and the viscous volume integral's fref
-// (parabolic_viscous.hpp:39-66 and :103-163 3D, :367-386 and :412-447 2D):
-// -F_v of each physical direction, then Ja^d_0 f_0 + Ja^d_1 f_1 (+ Ja^d_2 f_2)
-// from left to right, with no fused multiply-adds.  The interface value
-// (-F_v) . (sign(J) Ja^d) of a face node is sign(J) times this sum to the
-// bit (scaling by a sign is exact), so the RHS kernel needs nothing more.
-template <int DIM, int N, bool NODAL>
-__global__ void __launch_bounds__(256) viscous_node_flux_kernel(const KParams p) {
-  constexpr int NV = DIM + 2, NN = Shape<DIM, N>::NN;
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= p.E * NN) return;
-  const int64_t e = i / NN;
-  const int node = (int)(i % NN);
-  double s[NV], q[NV], g[DIM][NV], fv[DIM][NV];
-#pragma unroll
-  for (int v = 0; v < NV; ++v) s[v] = p.u[(e * NV + v) * NN + node];
-  grad_vars<DIM>(p.gamma, s, q);
-#pragma unroll
-  for (int c = 0; c < DIM; ++c)
-#pragma unroll
-    for (int v = 0; v < NV; ++v) g[c][v] = p.grad[((e * DIM + c) * NV + v) * NN + node];
-#pragma unroll
-  for (int c = 0; c < DIM; ++c) viscous_flux_dir<DIM>(p.mu, p.kappa, q, g, c, fv[c]);
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) {
-    double ja[DIM];
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) ja[c] = metric<NODAL, DIM, NN>(p, e, d, c, node);
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-#pragma clang fp contract(off)
-      double r = ja[0] * -fv[0][v];
-#pragma unroll
-      for (int c = 1; c < DIM; ++c) r = r + ja[c] * -fv[c][v];
-      p.visc_out[((e * DIM + d) * NV + v) * NN + node] = r;
-    }
-  }
+    for (int v = 0; v < NV; ++v) g[c][v] = g[c][v] * ij;
+  // the node's gradient is complete: its viscous fluxes follow here, so the
+  // gradient itself never goes to memory
+  store_viscous_fluxes<DIM, N, NODAL>(p, e, a, q, g);
 }
 
 template <int DIM, int N>
@@ -514,11 +509,6 @@ hipError_t launch_parabolic_t(bool sc, const KParams& p, hipStream_t s) {
     if (!sc) {
       hipLaunchKernelGGL((gradient_node_kernel<DIM, N, NODAL>), dim3((unsigned)p.E),
                          dim3(Shape<DIM, N>::NN), 0, s, p);
-      hipError_t err = hipGetLastError();
-      if (err != hipSuccess) return err;
-      const int64_t nodes = p.E * Shape<DIM, N>::NN;
-      hipLaunchKernelGGL((viscous_node_flux_kernel<DIM, N, NODAL>),
-                         dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, s, p);
       return hipGetLastError();
     }
   }
