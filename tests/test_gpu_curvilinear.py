@@ -76,7 +76,7 @@ def _rhs(gs, u):
 
 @pytest.mark.parametrize("dim,N,cells", [(2, 2, (6, 5)), (2, 3, (5, 4)), (2, 5, (3, 3)),
                                          (2, 7, (2, 3)), (3, 2, (3, 3, 2)), (3, 3, (3, 2, 3)),
-                                         (3, 4, (2, 2, 2))])
+                                         (3, 4, (2, 2, 2)), (3, 7, (2, 1, 2))])
 def test_curvilinear_rhs(gpu, dim, N, cells):
     print(f"case curvilinear rhs dim={dim} N={N}", flush=True)
     oc, gs = warped_pair(dim, N, cells)
@@ -103,10 +103,11 @@ def test_curvilinear_shock_capturing(gpu, dim, N, cells):
     assert rel_l2(got, ref) <= 1e-12
 
 
-def test_curvilinear_navier_stokes(gpu):
-    print("case curvilinear ns", flush=True)
-    oc, gs = warped_pair(2, 3, (5, 4), eq=dg.NAVIER_STOKES)
-    u = oc.init_khi()
+@pytest.mark.parametrize("dim,N,cells", [(2, 3, (5, 4)), (2, 7, (2, 3)), (3, 7, (2, 1, 2))])
+def test_curvilinear_navier_stokes(gpu, dim, N, cells):
+    print(f"case curvilinear ns dim={dim} N={N}", flush=True)
+    oc, gs = warped_pair(dim, N, cells, eq=dg.NAVIER_STOKES)
+    u = oc.init_khi() if dim == 2 else oc.init_tgv()
     assert rel_l2(gpu_rhs(gs, u), oc.rhs(u)) <= 1e-12
 
 
