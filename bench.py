@@ -18,6 +18,7 @@ import math
 import os
 import re
 import statistics
+import subprocess
 import sys
 import time
 
@@ -58,41 +59,93 @@ def stage_bytes(E: int, N: int, geometry: str) -> float:
     return state * (1.0 + 2.0 / 3.0) + state + metrics
 
 
-def read_traffic(path: str):
-    """HBM bytes per dispatch of the fused kernel from a committed rocprofv3
-    --pmc summary (FETCH_SIZE / WRITE_SIZE in KiB, already scaled)."""
+def read_profile(path: str) -> dict:
+    """Figures of the fused kernel from a committed rocprofv3 summary: HBM
+    bytes per dispatch (FETCH_SIZE + WRITE_SIZE, calibrated), the FP64 FLOPs
+    per dispatch counted from the FP64 VALU instruction counters, and the
+    kernel's average duration in the kernel trace."""
     if not os.path.exists(path):
-        return None
+        return {}
     text = open(path).read()
-    f = re.search(r"fetch_bytes_per_dispatch:\s*([0-9.eE+]+)", text)
-    w = re.search(r"write_bytes_per_dispatch:\s*([0-9.eE+]+)", text)
-    if not (f and w):
-        return None
-    return float(f.group(1)) + float(w.group(1))
+
+    def num(key):
+        m = re.search(key + r":\s*([0-9.eE+-]+)", text)
+        return float(m.group(1)) if m else None
+
+    out = {}
+    f, w = num("fetch_bytes_per_dispatch"), num("write_bytes_per_dispatch")
+    if f is not None and w is not None:
+        out["traffic"] = f + w
+    if num("fp64_flop_per_dispatch") is not None:
+        out["counter_flops"] = num("fp64_flop_per_dispatch")
+    if num("kernel_avg_us") is not None:
+        out["kernel_ms"] = num("kernel_avg_us") * 1e-3
+    return out
 
 
-def cpu_baseline(N: int, cells: int, steps: int) -> dict:
-    """The CPU restatement of the reference (OpenMP over elements, one thread
-    per host core of this box's share) on a bounded sample of the workload."""
+def _host_description() -> dict:
+    """CPU model, sockets and cores of this host (lscpu), and the cores this
+    process may run on."""
+    info = {"model": None, "sockets": None, "cores_per_socket": None, "threads_per_core": None}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                v = v.strip()
+                info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    except (OSError, subprocess.SubprocessError):
+        pass
+    info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    return info
+
+
+def _cpu_baseline_child(N: int, cells: int, steps: int, warmup: int) -> None:
+    """Runs in a child process started by cpu_baseline() with the OpenMP
+    environment set before the runtime starts; prints one JSON line."""
     from oracle import oracle as O
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = int(os.environ["OMP_NUM_THREADS"])
     c = O.Case(3, N, (cells,) * 3, [0, 0, 0], [2 * math.pi] * 3, threads=threads)
     u = c.init_tgv()
     dt = 1e-4
-    u = c.ssprk3(u, dt, 1)  # warm-up step
+    for _ in range(warmup):
+        u = c.ssprk3(u, dt, 1)
     times = []
     for _ in range(steps):
         t0 = time.perf_counter()
         u = c.ssprk3(u, dt, 1)
         times.append(time.perf_counter() - t0)
-    t_step = statistics.median(times)
-    dof = c.E * c.ndofs
+    print(json.dumps({"times": times, "dof": c.E * c.ndofs}))
+
+
+def cpu_baseline(N: int, cells: int, steps: int, warmup: int) -> dict:
+    """The CPU restatement of the reference (OpenMP over elements, one thread
+    per core of this process's CPU share, OMP_PLACES=cores,
+    OMP_PROC_BIND=close) on the same workload: `warmup` untimed SSPRK3 steps,
+    then the median of `steps` steps.  It runs in a child process so that the
+    OpenMP placement variables take effect before the runtime starts."""
+    host = _host_description()
+    threads = int(os.environ.get("OMP_NUM_THREADS", host["affinity_cpus"]))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PLACES="cores", OMP_PROC_BIND="close")
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-child",
+           "--polydeg", str(N), "--cpu-cells", str(cells), "--cpu-steps", str(steps),
+           "--cpu-warmup", str(warmup)]
+    r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu baseline failed: {r.stderr[-2000:]}")
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    t_step = statistics.median(res["times"])
+    dof = res["dof"]
     return {"value": dof / (t_step / 3.0), "unit": "DOF-updates/s per SSPRK3 stage",
             "cores": threads, "kind": "port",
             "sample": f"TGV N={N} on {cells}^3 elements ({dof} nodes), median of {steps} "
-                      f"SSPRK3 steps after 1 warm-up, oracle/ restatement, OpenMP {threads} threads",
-            "ms_per_step": t_step * 1e3}
+                      f"SSPRK3 steps after {warmup} warm-up steps, oracle/ restatement, "
+                      f"OpenMP {threads} threads, OMP_PLACES=cores OMP_PROC_BIND=close",
+            "ms_per_step": t_step * 1e3, "step_ms": [t * 1e3 for t in res["times"]],
+            "host": host}
 
 
 def batch_measure(ncases: int = 256, steps: int = 100, seed: int = 2024) -> dict:
@@ -309,7 +362,12 @@ def main() -> int:
     ap.add_argument("--geometry", choices=["nodal", "affine"], default="nodal")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-cells", type=int, default=64)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-warmup", type=int, default=2)
+    ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--settle-s", type=float, default=0.5,
+                    help="after the warm-up steps, keep stepping untimed for this many "
+                         "seconds so that the timed window runs at steady clocks")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-ensemble and adjoint measurements")
     ap.add_argument("--ns-n7", action="store_true",
@@ -319,6 +377,9 @@ def main() -> int:
                          "(calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
 
+    if args.cpu_baseline_child:
+        _cpu_baseline_child(args.polydeg, args.cpu_cells, args.cpu_steps, args.cpu_warmup)
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: there is no multi-GPU path "
               "(the RHS would shard by element blocks with a face-trace halo exchange). "
@@ -341,12 +402,32 @@ def main() -> int:
     lam = 1.0 + math.sqrt(1.4 * (100.0 / 1.4 + 0.25) / 1.0)
     dt = 0.5 * (2 * math.pi / cells) / (3 * (2 * N + 1) * lam)
 
+    # Untimed warm-up: the W steps asked for, then further steps until the
+    # GPU has run stage launches for at least --settle-s seconds, so that the
+    # timed window sees the clocks the kernel sustains rather than the boost
+    # of a cold start.
     for _ in range(args.warmup):
         rk.step(u, dt)
     torch.cuda.synchronize()
+    settle_steps, t_settle = 0, time.perf_counter()
+    while time.perf_counter() - t_settle < args.settle_s:
+        for _ in range(10):
+            rk.step(u, dt)
+        settle_steps += 10
+        torch.cuda.synchronize()
+
+    # Timed window: K steps, timed by the host clock and by HIP events on the
+    # stream the kernels run on (torch's current stream, which the solver
+    # handle was created with).  Every launch in the window is the fused
+    # RHS + stage kernel, so the event time over 3 K launches is that
+    # kernel's average launch duration in the same window as ms_per_step.
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
+    ev0.record()
     for _ in range(args.steps):
         rk.step(u, dt)
+    ev1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     if not bool(torch.isfinite(u).all()):
@@ -354,23 +435,41 @@ def main() -> int:
         return 1
     ms_per_step = wall / args.steps * 1e3
     value = E * nn / (ms_per_step / 3.0 * 1e-3)
+    t_kernel_ms = ev0.elapsed_time(ev1) / (3 * args.steps)
 
-    # Dominant kernel: the fused RHS + stage kernel.  Average launch duration
-    # from HIP events on the stream it runs on (torch's current stream).
-    reps = max(6, 3 * min(args.steps, 10))
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record()
-    for i in range(reps // 3):
-        rk.step(u, dt)
-    ev1.record()
-    torch.cuda.synchronize()
-    t_kernel_ms = ev0.elapsed_time(ev1) / (3 * (reps // 3))
     flops = volume_flops(E, N)
-    achieved = flops / (t_kernel_ms * 1e-3) / 1e12
-    traffic = read_traffic(os.path.join(ROOT, "profiles", f"rhs_stage_tgv{cells}_n{N}_{args.geometry}_pmc.txt"))
     hbm_bytes = stage_bytes(E, N, args.geometry)
+    intensity = flops / hbm_bytes
+    ridge = FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
+    bound = "hbm" if intensity < ridge else "fp64"
+    prof = read_profile(os.path.join(ROOT, "profiles", f"rhs_stage_tgv{cells}_n{N}_{args.geometry}_pmc.txt"))
+    traffic = prof.get("traffic")
+    achieved_gbs = hbm_bytes / (t_kernel_ms * 1e-3) / 1e9
+    achieved_tf = flops / (t_kernel_ms * 1e-3) / 1e12
     kernel_name = ("rhs_node_kernel (node-parallel fused volume+surface+stage)" if N >= 7
                    else "rhs_kernel (fused volume+surface+stage)")
+    if bound == "hbm":
+        roof = {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved_gbs / HBM_PEAK_GBS}
+    else:
+        roof = {"bound": "fp64", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved_tf / FP64_PEAK_TFLOPS}
+    roof.update({
+        "traffic": traffic, "kernel": kernel_name, "kernel_ms": t_kernel_ms,
+        "bytes_per_launch": hbm_bytes, "flop_per_launch": flops,
+        "arithmetic_intensity": intensity, "ridge_flop_per_byte": ridge,
+        # the other roof, side by side: algorithmic volume FLOPs against FP64 peak
+        "fp64": {"achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                 "frac": achieved_tf / FP64_PEAK_TFLOPS,
+                 # FP64 VALU instructions counted by rocprofv3 (per dispatch, x 64 lanes,
+                 # FMA as 2) over this window's launch duration
+                 "counter_flop_per_launch": prof.get("counter_flops"),
+                 "counter_achieved": (None if prof.get("counter_flops") is None
+                                      else prof["counter_flops"] / (t_kernel_ms * 1e-3) / 1e12)},
+        "measured_GB_per_s": None if traffic is None else traffic / (t_kernel_ms * 1e-3) / 1e9,
+        # average duration of the same kernel in the committed rocprofv3 trace
+        "profile_kernel_ms": prof.get("kernel_ms"),
+    })
 
     out = {
         "metric": "FP64 DOF-updates/s per SSPRK3 stage",
@@ -388,19 +487,10 @@ def main() -> int:
         "config": {"workload": f"tgv3d_euler_{cells}^3_N{N}_splitform_llf_ssprk3",
                    "elements": E, "polydeg": N, "nodes": E * nn, "nvars": 5,
                    "geometry": args.geometry, "dt": dt},
+        "settle": {"steps": settle_steps, "min_s": args.settle_s},
         "dof_var_updates_per_s": value * 5,
         "adjoint_forward_step_ratio": None,
-        "roofline": {"bound": "mfma", "kernel": kernel_name,
-                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "flop_per_launch": flops, "kernel_ms": t_kernel_ms},
-        # the same launch seen as the surface/stage kernel it also is: its
-        # algorithmic bytes (and the PMC-measured bytes) against HBM peak
-        "hbm": {"kernel": kernel_name,
-                "achieved": hbm_bytes / (t_kernel_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": hbm_bytes / (t_kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                "bytes_per_launch": hbm_bytes,
-                "measured_GB_per_s": None if traffic is None else traffic / (t_kernel_ms * 1e-3) / 1e9},
+        "roofline": roof,
     }
     if not args.no_extras:
         out["batch"] = batch_measure()
@@ -420,7 +510,7 @@ def main() -> int:
         torch.cuda.synchronize()
         out["copy_calibration_bytes"] = a.numel() * 8
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(N, args.cpu_cells, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(N, args.cpu_cells, args.cpu_steps, args.cpu_warmup)
     print(json.dumps(out))
     return 0
 

@@ -1793,6 +1793,9 @@ static void apply_floor(const orc_case* c, double* u) {
     }
 }
 
+/* The floor alone on a state, as every stage applies it. */
+void orc_apply_floor(const orc_case* c, double* u) { apply_floor(c, u); }
+
 /* parallel_ma2 / parallel_ma3 (SSPRK3.hpp:12-143) and SSPRK3::step
  * (SSPRK3.hpp:159-181) */
 void orc_ssprk3(orc_case* c, double* u, double dt, int nsteps) {

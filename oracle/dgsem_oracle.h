@@ -111,6 +111,8 @@ void orc_set_boundary(orc_case* c, int face, int kind, const double* state,
                       const double* values);
 
 /* SSPRK3::step (SSPRK3.hpp:159-181), nsteps times with fixed dt. */
+/* the positivity floor of a stage update, applied to u in place */
+void orc_apply_floor(const orc_case* c, double* u);
 void orc_ssprk3(orc_case* c, double* u, double dt, int nsteps);
 
 /* Pointwise kernels exposed for unit tests. */

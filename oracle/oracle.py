@@ -73,6 +73,7 @@ def lib():
     L.orc_apply_jacobian.argtypes = [vp, _dp]
     L.orc_gradients.argtypes = [vp, _dp, _dp]
     L.orc_ssprk3.argtypes = [vp, _dp, C.c_double, C.c_int]
+    L.orc_apply_floor.argtypes = [vp, _dp]
     L.orc_ln_mean.restype = C.c_double
     L.orc_ln_mean.argtypes = [C.c_double, C.c_double]
     for name in ("orc_flux_chandrashekar", "orc_flux_llf", "orc_flux_hllc"):
@@ -274,6 +275,12 @@ class Case:
     def ssprk3(self, u, dt, nsteps):
         u = np.array(u, dtype=np.float64, order="C", copy=True)
         lib().orc_ssprk3(self._h, u, float(dt), int(nsteps))
+        return u
+
+    def apply_floor(self, u):
+        """The positivity floor of a stage update on a copy of u."""
+        u = np.array(u, dtype=np.float64, order="C", copy=True)
+        lib().orc_apply_floor(self._h, u)
         return u
 
     def kinetic_energy(self, u):
