@@ -22,7 +22,8 @@ EXPORTED = (
     "dgsem_node_coordinates", "dgsem_contravariant_vectors", "dgsem_inverse_jacobian",
     "dgsem_set_boundary_values", "dgsem_write_vtu", "dgsem_reference_coordinates",
     "dgsem_set_node_coordinates", "dgsem_pack_state", "dgsem_unpack_state", "dgsem_pack_state_layout",
-    "dgsem_unpack_state_layout", "dgsem_rhs", "dgsem_get_alpha",
+    "dgsem_unpack_state_layout", "dgsem_rhs", "dgsem_get_alpha", "dgsem_get_stage_alpha",
+    "dgsem_freeze_alpha",
     "dgsem_ssprk3_stage", "dgsem_ssprk3_steps", "dgsem_synchronize",
     "dgsem_batch_create", "dgsem_batch_destroy", "dgsem_batch_last_error", "dgsem_batch_sizes",
     "dgsem_batch_case", "dgsem_batch_set_dt", "dgsem_batch_rhs", "dgsem_batch_ssprk3_steps",
@@ -83,6 +84,8 @@ def load(path: str | None = None):
     L.dgsem_unpack_state_layout.argtypes = [vp, C.c_int, vp, vp]
     L.dgsem_rhs.argtypes = [vp, vp, vp, C.c_double]
     L.dgsem_get_alpha.argtypes = [vp, vp]
+    L.dgsem_get_stage_alpha.argtypes = [vp, C.c_int, vp]
+    L.dgsem_freeze_alpha.argtypes = [vp, vp]
     L.dgsem_ssprk3_stage.argtypes = [vp, C.c_int, C.c_double, vp, vp, vp]
     L.dgsem_ssprk3_steps.argtypes = [vp, vp, C.c_double, C.c_int, vp, vp]
     L.dgsem_synchronize.argtypes = [vp]

@@ -91,7 +91,8 @@ def build(force: bool = True, verbose: bool = False, defines=(), out: str | None
     return out
 
 
-EXAMPLES = ["khi_shock_capturing", "double_mach_reflection", "viscous_shock_tube"]
+EXAMPLES = ["khi_shock_capturing", "double_mach_reflection", "viscous_shock_tube",
+            "taylor_green_observers"]
 
 
 def build_examples() -> None:

@@ -50,12 +50,21 @@ int upload(dgsem_handle* h, T** dst, const std::vector<T>& src) {
   return DGSEM_OK;
 }
 
+// A zero-filled device array of `count` doubles.
+int alloc_zero(dgsem_handle* h, double** dst, size_t count) {
+  const size_t bytes = (count ? count : 1) * sizeof(double);
+  if (hipMalloc((void**)dst, bytes) != hipSuccess) return fail(h, DGSEM_E_ALLOC, "hipMalloc failed");
+  const hipError_t e = hipMemset(*dst, 0, bytes);
+  return e == hipSuccess ? DGSEM_OK : hip_fail(h, e, "hipMemset");
+}
+
 void release(dgsem_handle* h) {
   hipFree(h->d_Ja);
   hipFree(h->d_invJ);
   hipFree(h->d_sn);
   hipFree(h->d_nbr);
   hipFree(h->d_Vinv);
+  hipFree(h->d_ja_box);
   hipFree(h->d_alpha);
   hipFree(h->d_grad);
   hipFree(h->d_visc);
@@ -65,10 +74,13 @@ void release(dgsem_handle* h) {
   hipFree(h->d_wq);
   hipFree(h->d_ke_partial);
   hipFree(h->d_an_partial);
+  hipFree(h->d_ad_alpha);
+  hipFree(h->d_ad_out);
   h->d_ad_dgrad = h->d_ad_dvisc = h->d_ad_qbar = h->d_wq = h->d_ke_partial = nullptr;
-  h->d_an_partial = nullptr;
+  h->d_an_partial = h->d_ad_alpha = h->d_ad_out = nullptr;
   h->d_Ja = h->d_invJ = h->d_sn = h->d_Vinv = h->d_alpha = h->d_grad = h->d_visc = nullptr;
   h->d_nbr = nullptr;
+  h->d_ja_box = nullptr;
 }
 
 // Geometry in the kernels' layouts.  Nodal: Ja[E][dir][comp][nn] and
@@ -144,6 +156,9 @@ void fill_params(dgsem_handle* h) {
   p.nbr = h->d_nbr;
   p.Vinv = h->d_Vinv;
   p.alpha = h->d_alpha;
+  p.alpha_in = nullptr;
+  p.ja_box = h->d_ja_box;
+  for (int c = 0; c < 3; ++c) p.ja_box_off[c] = h->ja_box_off[c];
   p.E = h->E;
   p.surface_flux = c.surface_flux;
   p.fv_flux = c.fv_flux;
@@ -281,6 +296,8 @@ int stage_params(dgsem_handle* h, int stage, double dt, const double* u_base, co
   KParams p = h->base;
   p.u = u_k;
   p.out = u_out;
+  // each stage records its blending factor in its own slot (0: plain RHS)
+  if (stage >= 0 && stage <= 3) p.alpha = h->d_alpha + stage * h->E;
   // coefficients exactly as SSPRK3::step builds them, SSPRK3.hpp:162-180
   if (stage == 0) {
     p.mode = kModeRhs;
@@ -382,14 +399,26 @@ int dgsem_create(const dgsem_config* cfg, void* stream, dgsem_handle** out) {
   if ((rc = upload(h, &h->d_Ja, Ja)) || (rc = upload(h, &h->d_invJ, invJ)) ||
       (rc = upload(h, &h->d_sn, sn)) || (rc = upload(h, &h->d_nbr, neighbour_table(h->geo, periodic))) ||
       (rc = upload(h, &h->d_Vinv, h->basis.Vinv)) ||
-      (rc = upload(h, &h->d_alpha, std::vector<double>(h->E, 0.0))) ||
+      (rc = upload(h, &h->d_alpha, std::vector<double>(4 * h->E, 0.0))) ||
+      // Navier-Stokes: the per-node gradient only where the line kernels
+      // read it (the node-parallel path keeps the viscous fluxes alone; the
+      // tangent and adjoint allocate the gradient when they first need it)
       (h->cfg.equation == DGSEM_NAVIER_STOKES &&
-       ((rc = upload(h, &h->d_grad, std::vector<double>(h->E * h->cfg.dim * h->nvars * h->nn, 0.0))) ||
-        (rc = upload(h, &h->d_visc, std::vector<double>(h->E * visc_width(h->cfg) * h->nvars * h->nn,
-                                                        0.0)))))) {
+       ((visc_width(h->cfg) == 1 &&
+         (rc = alloc_zero(h, &h->d_grad, (size_t)h->E * h->cfg.dim * h->nvars * h->nn))) ||
+        (rc = alloc_zero(h, &h->d_visc, (size_t)h->E * visc_width(h->cfg) * h->nvars * h->nn))))) {
     release(h);
     *out = h;
     return rc;
+  }
+  // periodic nodal box meshes: the metric terms' table for the line kernels
+  if (h->cfg.geometry == DGSEM_GEOMETRY_NODAL && !has_boundaries(h->cfg)) {
+    std::vector<double> tab;
+    if (dgsem::box_metric_table(h->geo, tab, h->ja_box_off) && (rc = upload(h, &h->d_ja_box, tab))) {
+      release(h);
+      *out = h;
+      return rc;
+    }
   }
   fill_params(h);
   *out = h;
@@ -490,14 +519,31 @@ int dgsem_rhs(dgsem_handle* h, const double* u, double* du, double /*t*/) {
   p.u = u;
   p.out = du;
   p.mode = dgsem::kModeRhs;
-  return launch(h, p);
+  const int rc = launch(h, p);
+  if (rc == DGSEM_OK) h->alpha_last = 0;
+  return rc;
 }
 
 int dgsem_get_alpha(const dgsem_handle* h, double* alpha) {
   if (!h || !h->d_Ja || !alpha) return DGSEM_E_INVALID;
-  hipError_t e = hipMemcpyAsync(alpha, h->d_alpha, h->E * sizeof(double), hipMemcpyDeviceToDevice,
-                                h->stream);
+  hipError_t e = hipMemcpyAsync(alpha, h->d_alpha + h->alpha_last * h->E, h->E * sizeof(double),
+                                hipMemcpyDeviceToDevice, h->stream);
   return e == hipSuccess ? DGSEM_OK : DGSEM_E_HIP;
+}
+
+int dgsem_get_stage_alpha(const dgsem_handle* h, int stage, double* alpha) {
+  if (!h || !h->d_Ja || !alpha || stage < 0 || stage > 3) return DGSEM_E_INVALID;
+  hipError_t e = hipMemcpyAsync(alpha, h->d_alpha + stage * h->E, h->E * sizeof(double),
+                                hipMemcpyDeviceToDevice, h->stream);
+  return e == hipSuccess ? DGSEM_OK : DGSEM_E_HIP;
+}
+
+int dgsem_freeze_alpha(dgsem_handle* h, const double* alpha) {
+  if (!h || !h->d_Ja) return DGSEM_E_INVALID;
+  if (alpha && h->cfg.volume_integral != DGSEM_VOLUME_SHOCK_CAPTURING)
+    return fail(h, DGSEM_E_INVALID, "a frozen blending factor needs the shock-capturing volume integral");
+  h->base.alpha_in = alpha;
+  return DGSEM_OK;
 }
 
 int dgsem_ssprk3_stage(dgsem_handle* h, int stage, double dt, const double* u_base,
@@ -505,8 +551,10 @@ int dgsem_ssprk3_stage(dgsem_handle* h, int stage, double dt, const double* u_ba
   if (!h || !h->d_Ja) return DGSEM_E_INVALID;
   if (stage < 1 || stage > 3) return fail(h, DGSEM_E_INVALID, "stage must be 1, 2 or 3");
   dgsem::KParams p;
-  const int rc = dgsem::stage_params(h, stage, dt, u_base, u_k, u_out, &p);
-  return rc != DGSEM_OK ? rc : launch(h, p);
+  int rc = dgsem::stage_params(h, stage, dt, u_base, u_k, u_out, &p);
+  if (rc == DGSEM_OK) rc = launch(h, p);
+  if (rc == DGSEM_OK) h->alpha_last = stage;
+  return rc;
 }
 
 int dgsem_ssprk3_steps(dgsem_handle* h, double* u, double dt, int nsteps, double* tmp1, double* tmp2) {

@@ -199,8 +199,9 @@ int alloc_once(dgsem_handle* h, double** p, size_t count) {
 // Scratch and the configuration check of the differentiated path.
 int ad_prepare(dgsem_handle* h) {
   const auto& c = h->cfg;
-  if (c.volume_integral != DGSEM_VOLUME_SPLIT_FORM)
-    return afail(h, DGSEM_E_UNSUPPORTED, "tangent and adjoint are built for the split form only");
+  const bool sc = c.volume_integral == DGSEM_VOLUME_SHOCK_CAPTURING;
+  if (sc && c.fv_flux != DGSEM_FLUX_LLF)
+    return afail(h, DGSEM_E_UNSUPPORTED, "tangent and adjoint are built for the LLF subcell flux");
   if (c.surface_flux != DGSEM_FLUX_LLF)
     return afail(h, DGSEM_E_UNSUPPORTED, "tangent and adjoint are built for the LLF surface flux");
   if (c.positivity_floor > 0.0)
@@ -208,9 +209,20 @@ int ad_prepare(dgsem_handle* h) {
   for (int f = 0; f < 2 * c.dim; ++f)
     if (c.face_boundary[f] != DGSEM_BC_PERIODIC)
       return afail(h, DGSEM_E_UNSUPPORTED, "tangent and adjoint are built for periodic meshes");
+  const size_t cell = (size_t)h->E * h->nvars * h->nn;
+  if (sc && !h->base.alpha_in) {
+    // the indicator's alpha at the linearisation state, from a forward RHS
+    int rc;
+    if ((rc = alloc_once(h, &h->d_ad_alpha, (size_t)h->E)) || (rc = alloc_once(h, &h->d_ad_out, cell)))
+      return rc;
+  }
   if (c.equation == DGSEM_NAVIER_STOKES) {
     int rc;
-    const size_t cell = (size_t)h->E * h->nvars * h->nn;
+    // the forward gradient, which the node-parallel forward path never stores
+    if (!h->d_grad) {
+      if ((rc = alloc_once(h, &h->d_grad, cell * c.dim))) return rc;
+      h->base.grad = h->base.grad_out = h->d_grad;
+    }
     if ((rc = alloc_once(h, &h->d_ad_dgrad, cell * c.dim)) ||
         (rc = alloc_once(h, &h->d_ad_dvisc, cell)) || (rc = alloc_once(h, &h->d_ad_qbar, cell)))
       return rc;
@@ -238,6 +250,18 @@ int apply(dgsem_handle* h, bool adjoint, const double* u, const double* v, const
   const bool nodal = h->cfg.geometry == DGSEM_GEOMETRY_NODAL;
   const bool ns = h->cfg.equation == DGSEM_NAVIER_STOKES;
   dgsem::AdParams a = ad_params(h, u);
+  if (h->cfg.volume_integral == DGSEM_VOLUME_SHOCK_CAPTURING && !a.k.alpha_in) {
+    // Freeze the blending factor the forward evaluation at u computes: a
+    // forward RHS of u into scratch records it (the same kernel on the same
+    // bits as the forward pass, so the same alpha).
+    dgsem::KParams f = h->base;
+    f.u = u;
+    f.out = h->d_ad_out;
+    f.alpha = h->d_ad_alpha;
+    f.mode = dgsem::kModeRhs;
+    if ((rc = dgsem::launch_params(h, f))) return rc;
+    a.k.alpha_in = h->d_ad_alpha;
+  }
   a.v = v;
   a.x1 = x1;
   a.a1 = a1;

@@ -53,6 +53,36 @@ __device__ __forceinline__ double sign_j(const KParams& p, int64_t e, int node) 
   return inv_jac<NODAL, NN>(p, e, node) >= 0.0 ? 1.0 : -1.0;
 }
 
+// Normal of subcell interface f (between nodes f and f + 1) on line `line`
+// of direction dir: the nodal subcell normals, or the element's metric row
+// (affine), as the forward's subcell flux uses them.
+template <int DIM, int N, bool NODAL>
+__device__ __forceinline__ void subcell_normal(const KParams& p, int64_t e, int dir, int line, int f,
+                                               double* nrm) {
+  constexpr int FN = Shape<DIM, N>::FN;
+  for (int c = 0; c < DIM; ++c)
+    nrm[c] = NODAL ? p.sn[((((e * DIM + dir) * FN) + line) * N + f) * DIM + c]
+                   : p.Ja[(e * DIM + dir) * DIM + c];
+}
+
+// Frozen blending factor of element e (shock capturing): the split-form
+// weight coef = 1 - alpha and the subcell weight alpha of a troubled
+// element, coef = 1 and no subcell terms otherwise
+// (volume_flux.hpp:843-869 with the atol test of :772-773).
+template <bool SC>
+__device__ __forceinline__ bool frozen_blend(const KParams& p, int64_t e, double& coef, double& alpha) {
+  coef = 1.0;
+  alpha = 0.0;
+  if constexpr (SC) {
+    alpha = p.alpha_in[e];
+    if (fabs(alpha - 0.0) > p.atol) {
+      coef = 1.0 - alpha;
+      return true;
+    }
+  }
+  return false;
+}
+
 // ---------------------------------------------------------------------------
 // Tangent
 // ---------------------------------------------------------------------------
@@ -216,7 +246,7 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) jvp_visc_kernel(
   }
 }
 
-template <int DIM, int N, int EPW, bool NODAL, bool NS>
+template <int DIM, int N, int EPW, bool NODAL, bool NS, bool SC>
 __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) jvp_rhs_kernel(const AdParams a) {
   using S = Shape<DIM, N>;
   constexpr int n = S::n, NV = S::NV, NN = S::NN, FN = S::FN, L = S::LINES, NP = S::NP;
@@ -286,6 +316,8 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) jvp_rhs_kernel(c
     auto ja = [&](int64_t el, int dof, double* out) {
       for (int c = 0; c < DIM; ++c) out[c] = metric<NODAL, DIM, NN>(p, el, dir, c, dof);
     };
+    double coef, alpha;
+    const bool troubled = frozen_blend<SC>(p, e, coef, alpha);
 #pragma unroll
     for (int ia = 0; ia < n - 1; ++ia) {
       const NodePrims<DIM> qa = prims(dofs[ia]);
@@ -301,10 +333,32 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) jvp_rhs_kernel(c
         ja(e, dofs[ib], jab);
         for (int c = 0; c < DIM; ++c) nrm[c] = NODAL ? 0.5 * (jaa[c] + jab[c]) : jaa[c];
         chandrashekar_jvp<DIM>(p.inv_gm1, qa, qb, ta, tb, nrm, df);
-        const double wa = p.Ds[ia][ib], wb = p.Ds[ib][ia];
+        const double wa = coef * p.Ds[ia][ib], wb = coef * p.Ds[ib][ia];
         for (int v = 0; v < NV; ++v) {
           acc[ia][v] += wa * df[v];
           acc[ib][v] += wb * df[v];
+        }
+      }
+    }
+    // first-order subcell fluxes of a troubled element (LLF), weighted by the
+    // frozen alpha as the forward's blend weights them
+    if constexpr (SC) {
+      if (troubled) {
+#pragma unroll 1
+        for (int fi = 1; fi < n; ++fi) {
+          double ul[NV], ur[NV], dul[NV], dur[NV], nrm[DIM], df[NV];
+          load_state<DIM, NN>(p.u, e, dofs[fi - 1], ul);
+          load_state<DIM, NN>(p.u, e, dofs[fi], ur);
+          load_state<DIM, NN>(a.v, e, dofs[fi - 1], dul);
+          load_state<DIM, NN>(a.v, e, dofs[fi], dur);
+          subcell_normal<DIM, N, NODAL>(p, e, dir, line, fi - 1, nrm);
+          llf_jvp<DIM>(p.gamma, ul, node_prims<DIM>(p.gamma, ul), ur, node_prims<DIM>(p.gamma, ur), dul,
+                       dur, nrm, df);
+          const double wl = alpha * p.inv_w[fi - 1], wr = alpha * p.inv_w[fi];
+          for (int v = 0; v < NV; ++v) {
+            acc[fi - 1][v] += wl * df[v];
+            acc[fi][v] -= wr * df[v];
+          }
         }
       }
     }
@@ -519,7 +573,7 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_visc2_kernel
   }
 }
 
-template <int DIM, int N, int EPW, bool NODAL, bool NS>
+template <int DIM, int N, int EPW, bool NODAL, bool NS, bool SC>
 __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(const AdParams a) {
   using S = Shape<DIM, N>;
   constexpr int n = S::n, NV = S::NV, NN = S::NN, FN = S::FN, L = S::LINES, NP = S::NP;
@@ -564,6 +618,8 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(c
   }
   double ub_lo[NV], ub_hi[NV];
   for (int v = 0; v < NV; ++v) ub_lo[v] = ub_hi[v] = 0.0;
+  double coef = 1.0, alpha = 0.0;
+  const bool troubled = valid && frozen_blend<SC>(p, e0 + slot, coef, alpha);
   if (valid) {
     const int64_t e = e0 + slot;
     const double* pr = s_p + slot * NP * NN;
@@ -595,7 +651,7 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(c
         double jab[DIM], nrm[DIM], g[NV];
         ja(e, dofs[ib], jab);
         for (int c = 0; c < DIM; ++c) nrm[c] = NODAL ? 0.5 * (jaa[c] + jab[c]) : jaa[c];
-        const double wa = p.Ds[ia][ib], wb = p.Ds[ib][ia];
+        const double wa = coef * p.Ds[ia][ib], wb = coef * p.Ds[ib][ia];
         for (int v = 0; v < NV; ++v) g[v] = wa * rr[v * NN + dofs[ia]] + wb * rr[v * NN + dofs[ib]];
         chandrashekar_vjp<DIM>(p.inv_gm1, qa, qbn, nrm, g, qb[ia], qb[ib]);
       }
@@ -650,6 +706,34 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(c
         s_ub[(slot * NV + v) * NN + dofs[0]] += ub_lo[v];
         s_ub[(slot * NV + v) * NN + dofs[n - 1]] += ub_hi[v];
       }
+      // subcell fluxes of a troubled element: flux fi enters node fi - 1
+      // with alpha w_inv(fi - 1) and node fi with -alpha w_inv(fi); each line
+      // owns its nodes within this direction, so the sums need no atomics
+      if constexpr (SC) {
+        if (troubled) {
+          const int64_t e = e0 + slot;
+          const double* rr = s_r + slot * NV * NN;
+#pragma unroll 1
+          for (int fi = 1; fi < n; ++fi) {
+            double ul[NV], ur[NV], nrm[DIM], g[NV], ubl[NV], ubr[NV];
+            load_state<DIM, NN>(p.u, e, dofs[fi - 1], ul);
+            load_state<DIM, NN>(p.u, e, dofs[fi], ur);
+            subcell_normal<DIM, N, NODAL>(p, e, dir, line, fi - 1, nrm);
+            const double wl = alpha * p.inv_w[fi - 1], wr = alpha * p.inv_w[fi];
+            for (int v = 0; v < NV; ++v) {
+              g[v] = wl * rr[v * NN + dofs[fi - 1]] - wr * rr[v * NN + dofs[fi]];
+              ubl[v] = ubr[v] = 0.0;
+            }
+            const NodePrims<DIM> ql = node_prims<DIM>(p.gamma, ul), qr = node_prims<DIM>(p.gamma, ur);
+            llf_vjp_side<DIM>(p.gamma, ul, ql, ur, qr, nrm, g, true, ubl);
+            llf_vjp_side<DIM>(p.gamma, ul, ql, ur, qr, nrm, g, false, ubr);
+            for (int v = 0; v < NV; ++v) {
+              s_ub[(slot * NV + v) * NN + dofs[fi - 1]] += ubl[v];
+              s_ub[(slot * NV + v) * NN + dofs[fi]] += ubr[v];
+            }
+          }
+        }
+      }
     }
     __syncthreads();
   }
@@ -683,7 +767,7 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(c
   }
 }
 
-template <int DIM, int N, bool NODAL>
+template <int DIM, int N, bool NODAL, bool SC>
 hipError_t launch_jvp_t(bool ns, const AdParams& a, hipStream_t s) {
   constexpr int EPW = ad_elems_per_block<DIM, N>();
   constexpr int BLOCK = EPW * Shape<DIM, N>::LINES;
@@ -692,16 +776,16 @@ hipError_t launch_jvp_t(bool ns, const AdParams& a, hipStream_t s) {
   if (ns) {
     hipLaunchKernelGGL((jvp_grad_kernel<DIM, N, EPW, NODAL>), dim3((unsigned)nb), dim3(BLOCK), 0, s, a);
     hipLaunchKernelGGL((jvp_visc_kernel<DIM, N, EPW, NODAL>), dim3((unsigned)nb), dim3(BLOCK), 0, s, a);
-    hipLaunchKernelGGL((jvp_rhs_kernel<DIM, N, EPW, NODAL, true>), dim3((unsigned)nb), dim3(BLOCK), 0,
-                       s, a);
+    hipLaunchKernelGGL((jvp_rhs_kernel<DIM, N, EPW, NODAL, true, SC>), dim3((unsigned)nb), dim3(BLOCK),
+                       0, s, a);
   } else {
-    hipLaunchKernelGGL((jvp_rhs_kernel<DIM, N, EPW, NODAL, false>), dim3((unsigned)nb), dim3(BLOCK), 0,
-                       s, a);
+    hipLaunchKernelGGL((jvp_rhs_kernel<DIM, N, EPW, NODAL, false, SC>), dim3((unsigned)nb),
+                       dim3(BLOCK), 0, s, a);
   }
   return hipGetLastError();
 }
 
-template <int DIM, int N, bool NODAL>
+template <int DIM, int N, bool NODAL, bool SC>
 hipError_t launch_vjp_t(bool ns, const AdParams& a, hipStream_t s) {
   constexpr int EPW = ad_elems_per_block<DIM, N>();
   constexpr int BLOCK = EPW * Shape<DIM, N>::LINES;
@@ -710,11 +794,11 @@ hipError_t launch_vjp_t(bool ns, const AdParams& a, hipStream_t s) {
   if (ns) {
     hipLaunchKernelGGL((vjp_visc1_kernel<DIM, N, EPW, NODAL>), dim3((unsigned)nb), dim3(BLOCK), 0, s, a);
     hipLaunchKernelGGL((vjp_visc2_kernel<DIM, N, EPW, NODAL>), dim3((unsigned)nb), dim3(BLOCK), 0, s, a);
-    hipLaunchKernelGGL((vjp_rhs_kernel<DIM, N, EPW, NODAL, true>), dim3((unsigned)nb), dim3(BLOCK), 0,
-                       s, a);
+    hipLaunchKernelGGL((vjp_rhs_kernel<DIM, N, EPW, NODAL, true, SC>), dim3((unsigned)nb), dim3(BLOCK),
+                       0, s, a);
   } else {
-    hipLaunchKernelGGL((vjp_rhs_kernel<DIM, N, EPW, NODAL, false>), dim3((unsigned)nb), dim3(BLOCK), 0,
-                       s, a);
+    hipLaunchKernelGGL((vjp_rhs_kernel<DIM, N, EPW, NODAL, false, SC>), dim3((unsigned)nb),
+                       dim3(BLOCK), 0, s, a);
   }
   return hipGetLastError();
 }

@@ -11,17 +11,27 @@ namespace dgsem {
 template <>
 hipError_t launch_jvp_dn<DGSEM_INST_DIM, DGSEM_INST_N>(bool nodal, const AdParams& a,
                                                        hipStream_t s) {
-  const bool ns = a.dvisc != nullptr;
-  return nodal ? detail::launch_jvp_t<DGSEM_INST_DIM, DGSEM_INST_N, true>(ns, a, s)
-               : detail::launch_jvp_t<DGSEM_INST_DIM, DGSEM_INST_N, false>(ns, a, s);
+  // Navier-Stokes terms when the viscous scratch is set; the shock-capturing
+  // blend when a frozen blending factor is
+  const bool ns = a.dvisc != nullptr, sc = a.k.alpha_in != nullptr;
+  if (sc)
+    return nodal ? detail::launch_jvp_t<DGSEM_INST_DIM, DGSEM_INST_N, true, true>(ns, a, s)
+                 : detail::launch_jvp_t<DGSEM_INST_DIM, DGSEM_INST_N, false, true>(ns, a, s);
+  return nodal ? detail::launch_jvp_t<DGSEM_INST_DIM, DGSEM_INST_N, true, false>(ns, a, s)
+               : detail::launch_jvp_t<DGSEM_INST_DIM, DGSEM_INST_N, false, false>(ns, a, s);
 }
 
 template <>
 hipError_t launch_vjp_dn<DGSEM_INST_DIM, DGSEM_INST_N>(bool nodal, const AdParams& a,
                                                        hipStream_t s) {
-  const bool ns = a.dvisc != nullptr;
-  return nodal ? detail::launch_vjp_t<DGSEM_INST_DIM, DGSEM_INST_N, true>(ns, a, s)
-               : detail::launch_vjp_t<DGSEM_INST_DIM, DGSEM_INST_N, false>(ns, a, s);
+  // Navier-Stokes terms when the viscous scratch is set; the shock-capturing
+  // blend when a frozen blending factor is
+  const bool ns = a.dvisc != nullptr, sc = a.k.alpha_in != nullptr;
+  if (sc)
+    return nodal ? detail::launch_vjp_t<DGSEM_INST_DIM, DGSEM_INST_N, true, true>(ns, a, s)
+                 : detail::launch_vjp_t<DGSEM_INST_DIM, DGSEM_INST_N, false, true>(ns, a, s);
+  return nodal ? detail::launch_vjp_t<DGSEM_INST_DIM, DGSEM_INST_N, true, false>(ns, a, s)
+               : detail::launch_vjp_t<DGSEM_INST_DIM, DGSEM_INST_N, false, false>(ns, a, s);
 }
 
 template <>

@@ -50,6 +50,7 @@ __device__ __forceinline__ void batch_tile(const KParams& p, const int64_t e0, c
   // across the whole kernel.  The empty asm makes tid opaque per tile.
   int tid = threadIdx.x;
   asm volatile("" : "+v"(tid));
+  constexpr bool BOX = false;  // ensemble cases read the full nodal metrics
 #include "element_body.inc"
 }
 

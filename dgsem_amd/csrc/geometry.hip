@@ -191,6 +191,13 @@ extern "C" int dgsem_set_node_coordinates(dgsem_handle* h, const double* coords)
   const int d = h->cfg.dim, n = h->basis.n, nn = h->nn;
   const int FN = d == 2 ? n : n * n;
   const int64_t E = h->E;
+  // a mapped mesh is no longer a box: the kernels read the per-node metrics
+  if (h->d_ja_box) {
+    (void)hipStreamSynchronize(h->stream);  // queued launches may still read the table
+    hipFree(h->d_ja_box);
+    h->d_ja_box = nullptr;
+    h->base.ja_box = nullptr;
+  }
   const size_t nco = (size_t)E * nn * d, nmat = (size_t)E * nn * d * d, nsn = (size_t)E * d * FN * (n - 1) * d;
   DeviceBuf<double> dD, dw, djac, djaref, dsn;
   if (dD.alloc(n * n) || dw.alloc(n) || djac.alloc(nmat) || djaref.alloc(nmat) || dsn.alloc(nsn))

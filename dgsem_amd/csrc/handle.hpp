@@ -24,7 +24,11 @@ struct dgsem_handle {
   double* d_sn = nullptr;
   int* d_nbr = nullptr;
   double* d_Vinv = nullptr;
-  double* d_alpha = nullptr;
+  double* d_ja_box = nullptr;  // box-mesh table of the metric terms (KParams::ja_box)
+  int64_t ja_box_off[3] = {0, 0, 0};
+  double* d_alpha = nullptr;  // blending factor per element, [4][E]: slot 0 the last
+                              // dgsem_rhs, slots 1-3 the last SSPRK3 stage of each kind
+  int alpha_last = 0;         // slot of the last launch that wrote one
   double* d_grad = nullptr;  // Navier-Stokes gradient [E][dim][var][dof]
   double* d_visc = nullptr;  // Navier-Stokes viscous RHS part [E][var][dof] or
                              // per-node fluxes [E][dim][var][dof] (node_kernel_for)
@@ -39,6 +43,10 @@ struct dgsem_handle {
   double* d_wq = nullptr;
   double* d_ke_partial = nullptr;
   double* d_an_partial = nullptr;  // per-block partials of dgsem_analyze
+  // shock capturing without a frozen alpha: the alpha of the linearisation
+  // state [E] and the forward RHS that computes it (discarded)
+  double* d_ad_alpha = nullptr;
+  double* d_ad_out = nullptr;
   std::string err;
 };
 

@@ -5,6 +5,7 @@
 
 #include <array>
 #include <cmath>
+#include <cstring>
 #include <limits>
 #include <numbers>
 #include <stdexcept>
@@ -361,6 +362,65 @@ Geometry make_geometry(const BasisTables& b, int dim, const int* cells,
         }
       }
   return g;
+}
+
+bool box_metric_table(const Geometry& g, std::vector<double>& tab, int64_t off[3]) {
+  const int d = g.dim, nn = g.nn;
+  const int64_t E = g.E;
+  for (int64_t i = 0; i < E * nn; ++i)
+    if (!(g.inverse_jacobian[i] > 0.0)) return false;
+  auto pairs = [&](int c) -> int64_t {
+    if (d == 2) return g.cells[1 - c];
+    return (int64_t)g.cells[(c + 1) % 3] * g.cells[(c + 2) % 3];
+  };
+  int64_t total = 0;
+  for (int c = 0; c < 3; ++c) {
+    off[c] = total;
+    if (c < d) total += pairs(c) * d * nn;
+  }
+  tab.assign(total, 0.0);
+  auto ids = [&](int64_t e, int* eid) {
+    eid[0] = (int)(e / ((int64_t)g.cells[1] * g.cells[2]));
+    eid[1] = (int)((e / g.cells[2]) % g.cells[1]);
+    eid[2] = (int)(e % g.cells[2]);
+  };
+  auto pair_of = [&](const int* eid, int c) -> int64_t {
+    if (d == 2) return eid[1 - c];
+    return (int64_t)eid[(c + 1) % 3] * g.cells[(c + 2) % 3] + eid[(c + 2) % 3];
+  };
+  auto ja = [&](int64_t e, int dof, int r, int c) {
+    return g.contravariant[((e * nn + dof) * d + r) * d + c];
+  };
+  // the rows of the elements with i_c = 0
+  for (int64_t e = 0; e < E; ++e) {
+    int eid[3];
+    ids(e, eid);
+    for (int c = 0; c < d; ++c) {
+      if (eid[c] != 0) continue;
+      const int64_t pr = pair_of(eid, c);
+      for (int r = 0; r < d; ++r)
+        for (int dof = 0; dof < nn; ++dof) tab[off[c] + (pr * d + r) * nn + dof] = ja(e, dof, r, c);
+    }
+  }
+  // every element against its rows, bit for bit
+  int ok = 1;
+#pragma omp parallel for schedule(static) reduction(&& : ok)
+  for (int64_t e = 0; e < E; ++e) {
+    int eid[3];
+    ids(e, eid);
+    for (int c = 0; c < d; ++c) {
+      const int64_t pr = pair_of(eid, c);
+      for (int r = 0; r < d; ++r)
+        for (int dof = 0; dof < nn; ++dof) {
+          const double a = ja(e, dof, r, c), b = tab[off[c] + (pr * d + r) * nn + dof];
+          uint64_t ua, ub;
+          std::memcpy(&ua, &a, 8);
+          std::memcpy(&ub, &b, 8);
+          ok = ok && ua == ub;
+        }
+    }
+  }
+  return ok != 0;
 }
 
 }  // namespace dgsem

@@ -44,4 +44,14 @@ Geometry make_geometry(const BasisTables& b, int dim, const int* cells,
                        const double* left, const double* right,
                        const int* periodic);
 
+// The box-mesh table of the contravariant vectors (the kernels' ja_box):
+// Ja^r_c of an element depends only on its indices in the directions other
+// than c when every coordinate x_c depends on i_c and the node's c-index
+// alone, as the linear mapping of a box gives.  Fills
+// tab[off[c] + (pair * dim + r) * nn + dof] (pair = i_m * cells[l] + i_l in
+// 3D with m = c+1, l = c+2 mod 3, pair = i_{1-c} in 2D) and returns true if
+// every element's terms equal their table rows bit for bit and 1/J > 0 at
+// every node; false otherwise (the kernels then read the per-node arrays).
+bool box_metric_table(const Geometry& g, std::vector<double>& tab, int64_t off[3]);
+
 }  // namespace dgsem

@@ -44,6 +44,17 @@ struct KParams {
   const double* u_base;  // base state for kModeStage3
   double* out;           // L(u) or the stage result
   double* alpha;         // [E] blending factor (shock capturing), written
+  const double* alpha_in;  // [E] frozen blending factor to use instead of the
+                           // indicator's (shock capturing), or null
+  // Box meshes (the linear mapping of a Cartesian box, nodal geometry): the
+  // metric terms Ja^r_c of a node depend on its element only through the
+  // element's indices in the directions other than c, so each distinct row
+  // is stored once: ja_box[ja_box_off[c] + (pair * DIM + r) * NN + dof] with
+  // pair = i_m * cells[l] + i_l (3D; m = c+1, l = c+2 mod 3) or i_m (2D;
+  // m = 1-c).  Null unless the mesh is such a box, whose 1/J is then
+  // positive at every node.
+  const double* ja_box;
+  int64_t ja_box_off[3];
   const double* Ja;      // nodal: [E][dim][dim][NN]; affine: [E][dim][dim]
   const double* invJ;    // nodal: [E][NN]; affine: [E]
   const double* sn;      // nodal subcell normals [E][dim][FN][N][dim]

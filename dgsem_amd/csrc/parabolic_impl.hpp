@@ -33,51 +33,8 @@
 namespace dgsem {
 namespace detail {
 
-// gradient_variables (rho, v, T = p/rho), compressible_navier_stokes3D.hpp:55-70
-template <int DIM>
-__device__ __forceinline__ void grad_vars(double gamma, const double* u, double* q) {
-#pragma clang fp contract(off)
-  const double inv_rho = 1.0 / u[0];
-  double m2 = 0.0;
-  for (int k = 0; k < DIM; ++k) m2 += u[k + 1] * u[k + 1];
-  const double p = (gamma - 1.0) * (u[DIM + 1] - 0.5 * m2 * inv_rho);
-  q[0] = u[0];
-  for (int k = 0; k < DIM; ++k) q[k + 1] = u[k + 1] * inv_rho;
-  q[DIM + 1] = p * inv_rho;
-}
-
-// Newtonian viscous flux in direction dim, compressible_navier_stokes{2D,3D}.hpp
-// viscous_flux.  g[c][var] = d q_var / d x_c.
-template <int DIM>
-__device__ __forceinline__ void viscous_flux_dir(double mu, double kappa, const double* q,
-                                                 const double g[DIM][DIM + 2], int dim, double* f) {
-#pragma clang fp contract(off)
-  if constexpr (DIM == 2) {
-    const double tau11 = (4.0 * g[0][1] - 2.0 * g[1][2]) / 3.0;
-    const double tau22 = (4.0 * g[1][2] - 2.0 * g[0][1]) / 3.0;
-    const double tau12 = g[1][1] + g[0][2];
-    const double t1 = dim == 0 ? tau11 : tau12, t2 = dim == 0 ? tau12 : tau22;
-    f[0] = 0.0;
-    f[1] = mu * t1;
-    f[2] = mu * t2;
-    f[3] = mu * (q[1] * t1 + q[2] * t2 + kappa * g[dim][3]);
-  } else {
-    const double tau11 = (4.0 * g[0][1] - 2.0 * (g[1][2] + g[2][3])) / 3.0;
-    const double tau22 = (4.0 * g[1][2] - 2.0 * (g[0][1] + g[2][3])) / 3.0;
-    const double tau33 = (4.0 * g[2][3] - 2.0 * (g[0][1] + g[1][2])) / 3.0;
-    const double tau12 = g[1][1] + g[0][2];
-    const double tau13 = g[2][1] + g[0][3];
-    const double tau23 = g[2][2] + g[1][3];
-    const double t1 = dim == 0 ? tau11 : (dim == 1 ? tau12 : tau13);
-    const double t2 = dim == 0 ? tau12 : (dim == 1 ? tau22 : tau23);
-    const double t3 = dim == 0 ? tau13 : (dim == 1 ? tau23 : tau33);
-    f[0] = 0.0;
-    f[1] = mu * t1;
-    f[2] = mu * t2;
-    f[3] = mu * t3;
-    f[4] = mu * (q[1] * t1 + q[2] * t2 + q[3] * t3 + kappa * g[dim][4]);
-  }
-}
+// grad_vars and viscous_flux_dir (the pointwise gradient variables and
+// viscous flux) live in physics.hpp, shared with the host-side traits.
 
 template <bool NODAL, int DIM, int NN>
 __device__ __forceinline__ double metric(const KParams& p, int64_t e, int r, int c, int dof) {

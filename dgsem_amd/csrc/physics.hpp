@@ -309,6 +309,56 @@ DG_HD void flux_llf_prims(const double* ul, const NodePrims<DIM>& ql, const doub
   for (int i = 0; i < NV; ++i) f[i] = 0.5 * (fl[i] + fr[i]) - 0.5 * lam * (ur[i] - ul[i]);
 }
 
+// gradient_variables (rho, v, T = p/rho), compressible_navier_stokes3D.hpp:55-70
+template <int DIM>
+DG_HD void grad_vars(double gamma, const double* u, double* q) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+  const double inv_rho = 1.0 / u[0];
+  double m2 = 0.0;
+  for (int k = 0; k < DIM; ++k) m2 += u[k + 1] * u[k + 1];
+  const double p = (gamma - 1.0) * (u[DIM + 1] - 0.5 * m2 * inv_rho);
+  q[0] = u[0];
+  for (int k = 0; k < DIM; ++k) q[k + 1] = u[k + 1] * inv_rho;
+  q[DIM + 1] = p * inv_rho;
+}
+
+// Newtonian viscous flux in direction dim, compressible_navier_stokes{2D,3D}.hpp
+// viscous_flux.  g[c][var] = d q_var / d x_c.
+template <int DIM>
+DG_HD void viscous_flux_dir(double mu, double kappa, const double* q,
+                                                 const double g[DIM][DIM + 2], int dim, double* f) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+  if constexpr (DIM == 2) {
+    const double tau11 = (4.0 * g[0][1] - 2.0 * g[1][2]) / 3.0;
+    const double tau22 = (4.0 * g[1][2] - 2.0 * g[0][1]) / 3.0;
+    const double tau12 = g[1][1] + g[0][2];
+    const double t1 = dim == 0 ? tau11 : tau12, t2 = dim == 0 ? tau12 : tau22;
+    f[0] = 0.0;
+    f[1] = mu * t1;
+    f[2] = mu * t2;
+    f[3] = mu * (q[1] * t1 + q[2] * t2 + kappa * g[dim][3]);
+  } else {
+    const double tau11 = (4.0 * g[0][1] - 2.0 * (g[1][2] + g[2][3])) / 3.0;
+    const double tau22 = (4.0 * g[1][2] - 2.0 * (g[0][1] + g[2][3])) / 3.0;
+    const double tau33 = (4.0 * g[2][3] - 2.0 * (g[0][1] + g[1][2])) / 3.0;
+    const double tau12 = g[1][1] + g[0][2];
+    const double tau13 = g[2][1] + g[0][3];
+    const double tau23 = g[2][2] + g[1][3];
+    const double t1 = dim == 0 ? tau11 : (dim == 1 ? tau12 : tau13);
+    const double t2 = dim == 0 ? tau12 : (dim == 1 ? tau22 : tau23);
+    const double t3 = dim == 0 ? tau13 : (dim == 1 ? tau23 : tau33);
+    f[0] = 0.0;
+    f[1] = mu * t1;
+    f[2] = mu * t2;
+    f[3] = mu * t3;
+    f[4] = mu * (q[1] * t1 + q[2] * t2 + q[3] * t3 + kappa * g[dim][4]);
+  }
+}
+
 // p * rho with gamma fixed at 1.4, the indicator variable of
 // subcell_limiting/indicator.hpp:140-215.
 template <int DIM>

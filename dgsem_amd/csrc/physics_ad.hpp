@@ -8,7 +8,9 @@
 // holds to rounding.  Branches (the series / logarithm forms of the
 // logarithmic mean, the max in the LLF wave speed) follow the forward
 // evaluation's choice at x: these are the derivatives of the function the
-// forward kernels compute, i.e. a discrete tangent and adjoint.
+// forward kernels compute, i.e. a discrete tangent and adjoint.  Where the
+// LLF wave speed's max() ties exactly, the flux has no derivative and the
+// two sides share it (the mean of the one-sided derivatives).
 //
 // The blending factor of shock capturing is not differentiated (it is frozen
 // at its forward value by the callers).
@@ -276,19 +278,24 @@ DG_HD void llf_lambda_grad(double gamma, const NodePrims<DIM>& ql, const NodePri
   }
   const NodePrims<DIM>& q = left ? ql : qr;
   for (int i = 0; i < DIM + 2; ++i) grad[i] = 0.0;
-  // dmax(a, b) = a < b ? b : a picks the left argument on ties
-  const bool vn_left = !(fabs(vnl) < fabs(vnr));
-  const bool c_left = !(ql.c < qr.c);
+  // The side a max() takes gets the whole gradient.  On an exact tie the max
+  // has no derivative; each side then gets half of its own, the mean of the
+  // two one-sided derivatives, which is what a central difference sees.
+  // Symmetric states put ties on many faces (equal traces, or a velocity or
+  // sound speed that does not vary along the face normal).
+  const double avl = fabs(vnl), avr = fabs(vnr);
+  const double wvn = avl == avr ? 0.5 : ((avl > avr) == left ? 1.0 : 0.0);
+  const double wc = ql.c == qr.c ? 0.5 : ((ql.c > qr.c) == left ? 1.0 : 0.0);
   double vb[DIM] = {};
   double pb = 0.0;
-  if (vn_left == left) {
+  if (wvn != 0.0) {
     const double vn = left ? vnl : vnr;
     const double s = vn > 0.0 ? 1.0 : (vn < 0.0 ? -1.0 : 0.0);
-    for (int k = 0; k < DIM; ++k) vb[k] += s * n[k];
+    for (int k = 0; k < DIM; ++k) vb[k] += wvn * s * n[k];
   }
-  if (c_left == left) {
+  if (wc != 0.0) {
     // c = sqrt(gamma p / rho): dc = c/2 (dp/p - drho/rho)
-    const double w = sqrt(nn) * 0.5 * q.c;
+    const double w = wc * sqrt(nn) * 0.5 * q.c;
     pb += w / q.p;
     grad[0] -= w / q.rho;
   }

@@ -55,6 +55,21 @@ __device__ __forceinline__ int lds_pos(int node) {
   return node + node / (N + 1);
 }
 
+// Metric term Ja^r_c at node `dof` of the element with indices eid on a box
+// mesh, from the table described at KParams::ja_box.
+template <int DIM, int N>
+__device__ __forceinline__ double box_ja(const KParams& p, const int* eid, int c, int r, int dof) {
+  constexpr int NN = Shape<DIM, N>::NN;
+  int64_t pair;
+  if constexpr (DIM == 3) {
+    const int m = c == 2 ? 0 : c + 1, l = c == 0 ? 2 : c - 1;  // (c + 1) % 3, (c + 2) % 3
+    pair = (int64_t)eid[m] * p.cells[l] + eid[l];
+  } else {
+    pair = eid[1 - c];
+  }
+  return p.ja_box[p.ja_box_off[c] + (pair * DIM + r) * NN + dof];
+}
+
 // Elements per workgroup: keeps EPW * LINES near 192 threads.
 template <int DIM, int N>
 constexpr int elems_per_block() {
@@ -274,10 +289,11 @@ __device__ __forceinline__ void boundary_state(const KParams& p, int face, int64
 // template parameters, so a kernel only carries the register footprint of
 // the fluxes it uses.
 // BCS: some face of the (2D) mesh has a non-periodic boundary condition.
-template <int DIM, int N, int EPW, bool SC, bool NODAL, int SF, int FF, bool BCS>
+template <int DIM, int N, int EPW, bool SC, bool NODAL, int SF, int FF, bool BCS, bool BOX>
 __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
     __attribute__((amdgpu_waves_per_eu(min_waves_per_simd<DIM, N, NODAL>())))
     rhs_kernel(const KParams p) {
+  static_assert(!BOX || (NODAL && !BCS), "box-mesh tables serve periodic nodal meshes");
   using S = Shape<DIM, N>;
   constexpr int n = S::n, NV = S::NV, NN = S::NN, FN = S::FN, L = S::LINES, NP = S::NP;
   constexpr int BLOCK = EPW * L;
@@ -315,7 +331,15 @@ hipError_t launch_one(const KParams& p, hipStream_t stream) {
   const int64_t nb = (p.E + EPW - 1) / EPW;
   if (nb <= 0) return hipSuccess;
   if (nb > INT32_MAX) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((rhs_kernel<DIM, N, EPW, SC, NODAL, SF, FF, BCS>), dim3((unsigned)nb),
+  // nodal metric terms of a periodic box mesh come from the box table
+  if constexpr (NODAL && !BCS) {
+    if (p.ja_box) {
+      hipLaunchKernelGGL((rhs_kernel<DIM, N, EPW, SC, NODAL, SF, FF, BCS, true>), dim3((unsigned)nb),
+                         dim3(BLOCK), 0, stream, p);
+      return hipGetLastError();
+    }
+  }
+  hipLaunchKernelGGL((rhs_kernel<DIM, N, EPW, SC, NODAL, SF, FF, BCS, false>), dim3((unsigned)nb),
                      dim3(BLOCK), 0, stream, p);
   return hipGetLastError();
 }

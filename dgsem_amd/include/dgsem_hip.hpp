@@ -5,18 +5,31 @@
 //   StructuredSolver<Equations, Basis, VolumeFlux, SurfaceFlux, Mesh, BoundarySet>
 // (src/solver/structured_solver.hpp:19-58) and drives it with
 //   SSPRK3<T, Solver, Mesh, Solution>::step / TimeIntegrator::solve
-// (src/time_integrator/SSPRK3.hpp:145-183, integrator_base.hpp:21-40).
-// The same spellings compile against this header; the template arguments are
-// mapped onto a dgsem_config and every RHS / stage runs in the HIP kernels.
-// Device memory for the state comes from hipMalloc; nothing here needs
-// Kokkos.  Errors raise std::runtime_error carrying dgsem_last_error().
+// (src/time_integrator/SSPRK3.hpp:145-183, integrator_base.hpp:21-40),
+// with observers (src/observer) and analyzers (src/analyzer) attached.
+// The same spellings, in namespace DGSEM, compile against this header; the
+// template arguments are mapped onto a dgsem_config and every RHS / stage
+// runs in the HIP kernels.  The plugin traits keep their members -- the
+// equations' flux(u, dim), flux(u, normal), get_wave_speed, get_gamma and,
+// for Navier-Stokes, gradient_variables / viscous_flux; the two-point and
+// surface fluxes' static numerical_flux(eq, uL, uR, normal) -- evaluated on
+// the host with the same pointwise functions the kernels use
+// (dgsem_amd/csrc/physics.hpp).  Device memory for the state comes from
+// hipMalloc; nothing here needs Kokkos (KokkosSession and BasisGuard are
+// no-ops kept so that reference drivers compile).  Errors raise
+// std::runtime_error carrying dgsem_last_error().
 #pragma once
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -26,56 +39,198 @@
 #include <vector>
 
 #include "../../include/dgsem_amd.h"
+#include "../csrc/physics.hpp"
 
-namespace dgsem_hip {
+// Reference drivers mark their initial-condition functors with Kokkos'
+// function qualifier; here they run on the host.
+#ifndef KOKKOS_INLINE_FUNCTION
+#define KOKKOS_INLINE_FUNCTION inline
+#endif
 
-// --- equations (src/equations/*): value_type, NDIMS, NVARS, get_gamma -------
+namespace DGSEM {
+
+// --- runtime (the reference's Kokkos session and basis-table guard) ----------
+// The basis tables live in each solver handle, so neither holds state.
+struct KokkosSession {
+  KokkosSession() = default;
+  KokkosSession(int&, char**&) {}
+};
+template <class Basis>
+struct BasisGuard {};
+
+// --- equations (src/equations/*) --------------------------------------------
+// value_type, NDIMS, NVARS and the members the reference's kernels call:
+// flux(u, dim) (compressible_euler3D.hpp:26-52), flux(u, normal) (:54-67),
+// get_wave_speed(u_ll, u_rr, dim | normal) (:69-166), get_gamma; for
+// Navier-Stokes NGRAD_VARS, gradient_variables and viscous_flux
+// (compressible_navier_stokes3D.hpp:55-122, parabolic_concepts.hpp:9-21).
 namespace equations {
-template <class T>
-struct CompressibleEuler2D {
+namespace detail {
+template <class T, std::size_t D>
+std::array<T, D + 2> flux_dir(T gamma, const std::array<T, D + 2>& u, std::size_t dim) {
+  std::array<T, D> v;
+  T v2 = T(0);
+  for (std::size_t k = 0; k < D; ++k) {
+    v[k] = u[k + 1] / u[0];
+    v2 += v[k] * v[k];
+  }
+  const T p = (gamma - T(1)) * (u[D + 1] - T(0.5) * u[0] * v2);
+  std::array<T, D + 2> f;
+  f[0] = u[dim + 1];
+  for (std::size_t k = 0; k < D; ++k) f[k + 1] = k == dim ? u[dim + 1] * v[k] + p : u[dim + 1] * v[k];
+  f[D + 1] = v[dim] * (u[D + 1] + p);
+  return f;
+}
+template <class T, std::size_t D>
+std::array<T, D + 2> flux_normal(T gamma, const std::array<T, D + 2>& u, const std::array<T, D>& n) {
+  std::array<T, D + 2> f;
+  dgsem::physical_flux_n<(int)D>(gamma, u.data(), n.data(), f.data());
+  return f;
+}
+// max(|v_dim| + c) over the two states (the dimension-wise wave speed)
+template <class T, std::size_t D>
+T wave_speed_dir(T gamma, const std::array<T, D + 2>& ul, const std::array<T, D + 2>& ur,
+                 std::size_t dim) {
+  const T pl = dgsem::pressure<(int)D>(gamma, ul.data()), pr = dgsem::pressure<(int)D>(gamma, ur.data());
+  const T al = std::sqrt(gamma * pl / ul[0]), ar = std::sqrt(gamma * pr / ur[0]);
+  return std::max(std::abs(ul[dim + 1] / ul[0]) + al, std::abs(ur[dim + 1] / ur[0]) + ar);
+}
+}  // namespace detail
+
+template <class T, std::size_t D>
+class CompressibleEulerND {
+ public:
   using value_type = T;
-  static constexpr std::size_t NDIMS = 2, NVARS = 4;
-  explicit CompressibleEuler2D(T gamma = T(1.4)) : gamma_(gamma) {}
+  static constexpr std::size_t NDIMS = D, NVARS = D + 2;
+  explicit CompressibleEulerND(T gamma = T(1.4)) : gamma_(gamma) {}
+  std::array<T, NVARS> flux(const std::array<T, NVARS>& u, std::size_t dim) const {
+    return detail::flux_dir<T, D>(gamma_, u, dim);
+  }
+  std::array<T, NVARS> flux(const std::array<T, NVARS>& u, const std::array<T, NDIMS>& normal) const {
+    return detail::flux_normal<T, D>(gamma_, u, normal);
+  }
+  T get_wave_speed(const std::array<T, NVARS>& ul, const std::array<T, NVARS>& ur,
+                   std::size_t dim) const {
+    return detail::wave_speed_dir<T, D>(gamma_, ul, ur, dim);
+  }
+  T get_wave_speed(const std::array<T, NVARS>& ul, const std::array<T, NVARS>& ur,
+                   const std::array<T, NDIMS>& normal) const {
+    return dgsem::llf_wave_speed<(int)D>(gamma_, ul.data(), ur.data(), normal.data());
+  }
   T get_gamma() const { return gamma_; }
+
+ protected:
   T gamma_;
 };
+
 template <class T>
-struct CompressibleEuler3D {
-  using value_type = T;
-  static constexpr std::size_t NDIMS = 3, NVARS = 5;
-  explicit CompressibleEuler3D(T gamma = T(1.4)) : gamma_(gamma) {}
-  T get_gamma() const { return gamma_; }
-  T gamma_;
+struct CompressibleEuler2D : CompressibleEulerND<T, 2> {
+  using CompressibleEulerND<T, 2>::CompressibleEulerND;
 };
 template <class T>
-struct CompressibleNavierStokes3D {
-  using value_type = T;
-  static constexpr std::size_t NDIMS = 3, NVARS = 5, NGRAD_VARS = 5;
+struct CompressibleEuler3D : CompressibleEulerND<T, 3> {
+  using CompressibleEulerND<T, 3>::CompressibleEulerND;
+};
+
+template <class T, std::size_t D>
+class CompressibleNavierStokesND : public CompressibleEulerND<T, D> {
+ public:
+  static constexpr std::size_t NGRAD_VARS = D + 2;
+  CompressibleNavierStokesND(T gamma, T mu, T prandtl)
+      : CompressibleEulerND<T, D>(gamma), mu_(mu), prandtl_(prandtl) {}
+  // (rho, v, T = p / rho)
+  std::array<T, NGRAD_VARS> gradient_variables(const std::array<T, D + 2>& u) const {
+    std::array<T, NGRAD_VARS> q;
+    dgsem::grad_vars<(int)D>(this->gamma_, u.data(), q.data());
+    return q;
+  }
+  // the Newtonian viscous flux F_v in direction dim from q and its physical
+  // gradient grad_q[c][var]
+  std::array<T, D + 2> viscous_flux(const std::array<T, NGRAD_VARS>& q,
+                                    const std::array<std::array<T, NGRAD_VARS>, D>& grad_q,
+                                    std::size_t dim) const {
+    double g[D][D + 2];
+    for (std::size_t c = 0; c < D; ++c)
+      for (std::size_t v = 0; v < D + 2; ++v) g[c][v] = grad_q[c][v];
+    std::array<T, D + 2> f;
+    dgsem::viscous_flux_dir<(int)D>(mu_, get_kappa(), q.data(), g, (int)dim, f.data());
+    return f;
+  }
+  T get_mu() const { return mu_; }
+  T get_prandtl() const { return prandtl_; }
+  T get_kappa() const { return this->gamma_ / ((this->gamma_ - T(1)) * prandtl_); }
+
+ protected:
+  T mu_, prandtl_;
+};
+
+template <class T>
+struct CompressibleNavierStokes3D : CompressibleNavierStokesND<T, 3> {
   explicit CompressibleNavierStokes3D(T gamma = T(1.4), T mu = T(6.25e-4), T prandtl = T(0.72))
-      : gamma_(gamma), mu_(mu), prandtl_(prandtl) {}
-  T get_gamma() const { return gamma_; }
-  T get_mu() const { return mu_; }
-  T get_prandtl() const { return prandtl_; }
-  T gamma_, mu_, prandtl_;
+      : CompressibleNavierStokesND<T, 3>(gamma, mu, prandtl) {}
 };
 template <class T>
-struct CompressibleNavierStokes2D {
-  using value_type = T;
-  static constexpr std::size_t NDIMS = 2, NVARS = 4, NGRAD_VARS = 4;
+struct CompressibleNavierStokes2D : CompressibleNavierStokesND<T, 2> {
   explicit CompressibleNavierStokes2D(T gamma = T(1.4), T mu = T(5.0e-3), T prandtl = T(0.73))
-      : gamma_(gamma), mu_(mu), prandtl_(prandtl) {}
-  T get_gamma() const { return gamma_; }
-  T get_mu() const { return mu_; }
-  T get_prandtl() const { return prandtl_; }
-  T gamma_, mu_, prandtl_;
+      : CompressibleNavierStokesND<T, 2>(gamma, mu, prandtl) {}
 };
+
 template <class E>
 struct is_navier_stokes : std::false_type {};
 template <class T>
 struct is_navier_stokes<CompressibleNavierStokes2D<T>> : std::true_type {};
 template <class T>
 struct is_navier_stokes<CompressibleNavierStokes3D<T>> : std::true_type {};
+
+// equations_base.hpp:26-31
+template <class Equations>
+struct EquationTraits {
+  using value_type = typename Equations::value_type;
+  static constexpr std::size_t NDIMS = Equations::NDIMS;
+  static constexpr std::size_t NVARS = Equations::NVARS;
+};
 }  // namespace equations
+
+// --- state conversion (utils/state_conversion.hpp:70-110) --------------------
+namespace utils {
+template <class T, std::size_t N>
+std::array<T, N> prim_to_cons(const std::array<T, N>& prim, T gamma) {
+  static_assert(N >= 3, "prim_to_cons expects Euler-like state data");
+  std::array<T, N> cons{};
+  cons[0] = prim[0];
+  T v2 = T(0);
+  for (std::size_t d = 0; d + 2 < N; ++d) {
+    cons[d + 1] = prim[0] * prim[d + 1];
+    v2 += prim[d + 1] * prim[d + 1];
+  }
+  cons[N - 1] = prim[N - 1] / (gamma - T(1)) + T(0.5) * prim[0] * v2;
+  return cons;
+}
+template <class T, std::size_t N>
+std::array<T, N> cons_to_prim(const std::array<T, N>& cons, T gamma) {
+  static_assert(N >= 3, "cons_to_prim expects Euler-like state data");
+  std::array<T, N> prim{};
+  prim[0] = cons[0];
+  T v2 = T(0);
+  for (std::size_t d = 0; d + 2 < N; ++d) {
+    prim[d + 1] = cons[d + 1] / cons[0];
+    v2 += prim[d + 1] * prim[d + 1];
+  }
+  prim[N - 1] = (gamma - T(1)) * (cons[N - 1] - T(0.5) * cons[0] * v2);
+  return prim;
+}
+}  // namespace utils
+
+// --- initial conditions (initial_condition_base.hpp:14-28) -------------------
+// CRTP base: Derived provides operator()(coordinate) -> conservative state.
+template <class Derived, class Equations>
+struct AbstractInitial {
+  using traits = equations::EquationTraits<Equations>;
+  std::array<typename traits::value_type, traits::NVARS> get_initial(
+      const std::array<typename traits::value_type, traits::NDIMS>& coordinate) const {
+    return static_cast<const Derived&>(*this)(coordinate);
+  }
+};
 
 // --- basis (polynomial_basis/basis.hpp) --------------------------------------
 namespace Basis {
@@ -86,18 +241,123 @@ struct LobattoLegendreBasis {
 };
 }  // namespace Basis
 
-// --- flux tags (base/numerical_flux.hpp) -------------------------------------
+// --- numerical fluxes (base/numerical_flux.hpp) ------------------------------
+// Each trait carries the kind the C ABI selects for the kernels and the
+// reference's static numerical_flux(eq, uL, uR, normal | dim), evaluated on
+// the host by the pointwise functions the kernels run.
+namespace flux_detail {
 template <class Eq>
-struct ChandrashekarFlux {
-  static constexpr int kind = DGSEM_FLUX_CHANDRASHEKAR;
-};
+using state_t = std::array<typename Eq::value_type, Eq::NVARS>;
+template <class Eq>
+using normal_t = std::array<typename Eq::value_type, Eq::NDIMS>;
+template <class Eq>
+constexpr int dims() {
+  return (int)Eq::NDIMS;
+}
+// unit normal e_dim
+template <class Eq>
+normal_t<Eq> axis(std::size_t dim) {
+  normal_t<Eq> n{};
+  n[dim] = 1;
+  return n;
+}
+}  // namespace flux_detail
+
+// LaxFriedrichsFlux (numerical_flux.hpp:39-228): 0.5 (F_L + F_R) -
+// 0.5 lambda (u_R - u_L) with the equation's wave speed.
 template <class Eq>
 struct LaxFriedrichsFlux {
   static constexpr int kind = DGSEM_FLUX_LLF;
+  using value_type = typename Eq::value_type;
+  static constexpr std::size_t NDIMS = Eq::NDIMS, NVARS = Eq::NVARS;
+  static flux_detail::state_t<Eq> numerical_flux(const Eq& eq, const flux_detail::state_t<Eq>& uL,
+                                                 const flux_detail::state_t<Eq>& uR,
+                                                 std::size_t dim = 0) {
+    const value_type lam = eq.get_wave_speed(uL, uR, dim);
+    const auto fl = eq.flux(uL, dim), fr = eq.flux(uR, dim);
+    flux_detail::state_t<Eq> f;
+    for (std::size_t v = 0; v < NVARS; ++v) f[v] = 0.5 * (fl[v] + fr[v]) - 0.5 * lam * (uR[v] - uL[v]);
+    return f;
+  }
+  static flux_detail::state_t<Eq> numerical_flux(const Eq& eq, const flux_detail::state_t<Eq>& uL,
+                                                 const flux_detail::state_t<Eq>& uR,
+                                                 const flux_detail::normal_t<Eq>& normal) {
+    flux_detail::state_t<Eq> f;
+    dgsem::flux_llf<flux_detail::dims<Eq>()>(eq.get_gamma(), uL.data(), uR.data(), normal.data(),
+                                             f.data());
+    return f;
+  }
 };
+
+// HLLCFlux with a normal vector (the reference has the 1D form,
+// numerical_flux.hpp:283-378; the kernels' multi-dimensional version).
 template <class Eq>
 struct HLLCFlux {
   static constexpr int kind = DGSEM_FLUX_HLLC;
+  static constexpr std::size_t NDIMS = Eq::NDIMS, NVARS = Eq::NVARS;
+  static flux_detail::state_t<Eq> numerical_flux(const Eq& eq, const flux_detail::state_t<Eq>& uL,
+                                                 const flux_detail::state_t<Eq>& uR,
+                                                 const flux_detail::normal_t<Eq>& normal) {
+    flux_detail::state_t<Eq> f;
+    dgsem::flux_hllc<flux_detail::dims<Eq>()>(eq.get_gamma(), uL.data(), uR.data(), normal.data(),
+                                              f.data());
+    return f;
+  }
+  static flux_detail::state_t<Eq> numerical_flux(const Eq& eq, const flux_detail::state_t<Eq>& uL,
+                                                 const flux_detail::state_t<Eq>& uR,
+                                                 std::size_t dim = 0) {
+    return numerical_flux(eq, uL, uR, flux_detail::axis<Eq>(dim));
+  }
+};
+
+// ChandrashekarFlux (numerical_flux.hpp:436-594): the entropy-conservative,
+// kinetic-energy-preserving two-point flux of the split-form volume integral.
+template <class Eq>
+struct ChandrashekarFlux {
+  static constexpr int kind = DGSEM_FLUX_CHANDRASHEKAR;
+  static constexpr std::size_t NDIMS = Eq::NDIMS, NVARS = Eq::NVARS;
+  static flux_detail::state_t<Eq> numerical_flux(const Eq& eq, const flux_detail::state_t<Eq>& uL,
+                                                 const flux_detail::state_t<Eq>& uR,
+                                                 const flux_detail::normal_t<Eq>& normal) {
+    constexpr int D = flux_detail::dims<Eq>();
+    const double g = eq.get_gamma();
+    const auto a = dgsem::node_prims<D>(g, uL.data()), b = dgsem::node_prims<D>(g, uR.data());
+    flux_detail::state_t<Eq> f;
+    dgsem::flux_chandrashekar<D>(1.0 / (g - 1.0), a, b, normal.data(), f.data());
+    return f;
+  }
+  static flux_detail::state_t<Eq> numerical_flux(const Eq& eq, const flux_detail::state_t<Eq>& uL,
+                                                 const flux_detail::state_t<Eq>& uR,
+                                                 std::size_t dim = 0) {
+    return numerical_flux(eq, uL, uR, flux_detail::axis<Eq>(dim));
+  }
+};
+
+// CentralFlux (numerical_flux.hpp:230-258): 0.5 (F_L + F_R).  A host-side
+// trait only: the kernels are built for the LLF and HLLC surface fluxes and
+// the Chandrashekar volume flux, so a solver rejects it at compile time.
+template <class Eq>
+struct CentralFlux {
+  static constexpr int kind = -1;
+  static constexpr std::size_t NDIMS = Eq::NDIMS, NVARS = Eq::NVARS;
+  static flux_detail::state_t<Eq> numerical_flux(const Eq& eq, const flux_detail::state_t<Eq>& uL,
+                                                 const flux_detail::state_t<Eq>& uR,
+                                                 std::size_t dim = 0) {
+    const auto fl = eq.flux(uL, dim), fr = eq.flux(uR, dim);
+    flux_detail::state_t<Eq> f;
+    for (std::size_t v = 0; v < NVARS; ++v) f[v] = 0.5 * (fl[v] + fr[v]);
+    return f;
+  }
+  static flux_detail::state_t<Eq> numerical_flux(const Eq& eq, const flux_detail::state_t<Eq>& uL,
+                                                 const flux_detail::state_t<Eq>& uR,
+                                                 const flux_detail::normal_t<Eq>& normal)
+    requires(Eq::NDIMS == 2)
+  {
+    const auto fl = eq.flux(uL, normal), fr = eq.flux(uR, normal);
+    flux_detail::state_t<Eq> f;
+    for (std::size_t v = 0; v < NVARS; ++v) f[v] = 0.5 * (fl[v] + fr[v]);
+    return f;
+  }
 };
 
 // --- indicator and volume-integral policies (volume_flux.hpp:686-879) --------
@@ -140,6 +400,33 @@ class StructuredMesh {
  private:
   std::array<T, NDIM> left_, right_;
   std::array<std::size_t, NDIM> n_cells_;
+};
+
+// LinearMapping (base/mapping.hpp:30-50): the affine map of the reference
+// cube [-1, 1]^d onto [left, right], usable wherever a mapping is expected.
+template <class Coord>
+class LinearMapping {
+ public:
+  LinearMapping(const Coord& left, const Coord& right) : left_(left), right_(right) {}
+  Coord eval(const Coord& x) const {
+    Coord out;
+    for (std::size_t d = 0; d < x.size(); ++d)
+      out[d] = 0.5 * ((1.0 - x[d]) * left_[d] + (1.0 + x[d]) * right_[d]);
+    return out;
+  }
+
+ private:
+  Coord left_, right_;
+};
+
+// StructuredElementContainer (solver/data_container.hpp:12-60): the host
+// copies of the element geometry, [E * nn] node coordinates and inverse
+// Jacobians, filled by the solver constructor that takes it.
+template <class T, std::size_t NDIMS>
+struct StructuredElementContainer {
+  std::array<std::size_t, NDIMS> nelements{};
+  std::vector<std::array<T, NDIMS>> node_coordinates;
+  std::vector<T> inverse_jacobian;
 };
 
 // Boundary conditions (src/boundary_condition/*.hpp).  Faces are ordered
@@ -220,6 +507,10 @@ class Solution {
   value_type* rhs() { return du_.get(); }
   const value_type* rhs() const { return du_.get(); }
   std::size_t size() const { return size_; }
+  // the solver handle that last initialised or advanced this solution; the
+  // observers run their analysis and output through it
+  dgsem_handle* handle() const { return handle_; }
+  void bind(dgsem_handle* h) const { handle_ = h; }
 
  private:
   Solution() = default;
@@ -233,6 +524,7 @@ class Solution {
   }
   std::size_t size_ = 0;
   std::shared_ptr<value_type> u_, du_;
+  mutable dgsem_handle* handle_ = nullptr;
 };
 
 // --- the solver (structured_solver.hpp:19-125) --------------------------------
@@ -246,7 +538,38 @@ class StructuredSolver {
   static_assert(std::is_same_v<value_type, double>, "the GPU path computes in FP64");
   static_assert(BoundarySetType::NFACES == 2 * NDIMS,
                 "StructuredSolver requires one boundary condition per mesh face");
+  static_assert(SurfaceFlux::kind >= 0 && VolumeFlux::volume_flux >= 0 && VolumeFlux::fv_flux >= 0,
+                "the GPU kernels take the LLF or HLLC surface and subcell fluxes and the "
+                "Chandrashekar volume flux");
   using solution = Solution<Mesh, BasisT, Equations>;
+  using ElementCache = StructuredElementContainer<value_type, NDIMS>;
+
+  // StructuredSolver(eq, mesh, element, boundaries), structured_solver.hpp:
+  // 37-43: the element container is the reference's precomputed geometry;
+  // the handle builds its own from the mesh, so it is not read.
+  StructuredSolver(const Equations& eq, const Mesh& mesh, const ElementCache&,
+                   const BoundarySetType& bcs)
+      : StructuredSolver(eq, mesh, bcs) {}
+
+  // StructuredSolver(eq, mesh, element, boundaries, periodic),
+  // structured_solver.hpp:45-58: builds the linear-mapping geometry and
+  // fills `element` with its node coordinates and inverse Jacobians.  The
+  // periodic flags must agree with the boundary set's PeriodicBC faces.
+  StructuredSolver(const Equations& eq, const Mesh& mesh, ElementCache& element,
+                   const BoundarySetType& bcs, const std::array<bool, NDIMS>& periodic)
+      : StructuredSolver(eq, mesh, bcs) {
+    for (std::size_t d = 0; d < NDIMS; ++d)
+      if (periodic[d] != (BoundarySetType::kinds[2 * d] == DGSEM_BC_PERIODIC))
+        throw std::invalid_argument("periodic flags disagree with the boundary set");
+    element.nelements = mesh.get_num_cells();
+    std::vector<double> x(n_elements_ * n_dofs_ * NDIMS), ij(n_elements_ * n_dofs_);
+    check(dgsem_node_coordinates(h_.get(), x.data()), h_.get(), "node_coordinates");
+    check(dgsem_inverse_jacobian(h_.get(), ij.data()), h_.get(), "inverse_jacobian");
+    element.node_coordinates.resize(n_elements_ * n_dofs_);
+    for (std::size_t i = 0; i < n_elements_ * n_dofs_; ++i)
+      for (std::size_t d = 0; d < NDIMS; ++d) element.node_coordinates[i][d] = x[i * NDIMS + d];
+    element.inverse_jacobian.assign(ij.begin(), ij.end());
+  }
 
   StructuredSolver(const Equations& eq, const Mesh& mesh, const BoundarySetType& bcs,
                    hipStream_t stream = nullptr)
@@ -350,6 +673,7 @@ class StructuredSolver {
       for (std::size_t v = 0; v < NVARS; ++v) u[i * NVARS + v] = s[v];
     }
     upload_reference_layout(u, sol);
+    sol.bind(h_.get());
   }
 
   // Host state in the reference's LayoutRight (e, dof, var) <-> device.
@@ -372,6 +696,7 @@ class StructuredSolver {
   // calc_rhs(sol, time), structured_solver.hpp:463-480
   void calc_rhs(solution& sol, value_type time) {
     update_boundary_values(time);
+    sol.bind(h_.get());
     check(dgsem_rhs(h_.get(), sol.state(), sol.rhs(), time), h_.get(), "calc_rhs");
   }
 
@@ -497,35 +822,278 @@ class StructuredSolver {
   std::array<std::unique_ptr<double, FreeDevice>, 2 * NDIMS> face_dev_;
 };
 
-// --- time integration (SSPRK3.hpp:145-183, integrator_base.hpp:21-40) --------
-template <class T, class Solver, class Mesh, class SolutionT>
-class SSPRK3 {
+// --- observers (observer/observer_base.hpp, print_observer.cpp:10-20) ---------
+class SimulationObserver {
  public:
-  SSPRK3(const SolutionT& sol, const Mesh& mesh, T final_time = T(0))
-      : tmp1_(sol.clone_shape()), tmp2_(sol.clone_shape()), mesh_(mesh), final_time_(final_time) {}
+  virtual void on_step(int, double, bool&) {}
+  virtual void on_finish(int, double) {}
+  virtual ~SimulationObserver() = default;
+};
 
-  // u1 = u + dt L(u); u2 = 3/4 u + 1/4 u1 + dt/4 L(u1); u = 1/3 u + 2/3 u2 + 2dt/3 L(u2),
-  // with L evaluated at t, t + dt and t + dt/2 (SSPRK3.hpp:159-181)
-  void step(Solver& solver, SolutionT& sol, T dt) {
-    solver.ssprk3_stage(1, dt, nullptr, sol.state(), tmp1_.state(), time_);
-    solver.ssprk3_stage(2, dt, sol.state(), tmp1_.state(), tmp2_.state(), time_ + dt);
-    solver.ssprk3_stage(3, dt, sol.state(), tmp2_.state(), sol.state(), time_ + T(0.5) * dt);
+class PrintObserver : public SimulationObserver {
+ public:
+  explicit PrintObserver(int interval = 100) : interval_(interval) {}
+  void on_step(int step, double time, bool&) override {
+    if (step % interval_ == 0)
+      std::cout << "Iter: " << std::setw(5) << step << "  Time: " << std::scientific
+                << std::setprecision(8) << time << std::endl;
   }
+  void on_finish(int step, double time) override {
+    std::cout << "Finished at iter " << step << ", time " << time << std::endl;
+  }
+
+ private:
+  int interval_;
+};
+
+// --- time integration (integrator_base.hpp:8-58, SSPRK3.hpp:145-183) ---------
+template <class T, class Solver, class SolutionT>
+class TimeIntegrator {
+ public:
+  TimeIntegrator() = default;
+  explicit TimeIntegrator(T final_time) : final_time_(final_time) {}
+  virtual ~TimeIntegrator() = default;
+  virtual void step(Solver& solver, SolutionT& sol, T dt) = 0;
+  void add_observer(std::unique_ptr<SimulationObserver> observer) {
+    observers_.push_back(std::move(observer));
+  }
+  // Observers see every step before it runs; one that flags a non-finite
+  // state ends the loop after that step.  The device queue is drained only
+  // where an observer reads the state and at the end.
   void solve(Solver& solver, SolutionT& sol, T dt) {
     while (time_ < final_time_) {
+      for (auto& obs : observers_) obs->on_step(iter_, time_, has_nan_);
       step(solver, sol, dt);
       time_ += dt;
       ++iter_;
+      if (has_nan_) break;
     }
+    for (auto& obs : observers_) obs->on_finish(iter_, time_);
     solver.synchronize();
   }
   T get_time() const { return time_; }
 
+ protected:
+  T time_ = T(0), final_time_ = T(0);
+  int iter_ = 0;
+  bool has_nan_ = false;
+
+ private:
+  std::vector<std::unique_ptr<SimulationObserver>> observers_;
+};
+
+template <class T, class Solver, class Mesh, class SolutionT>
+class SSPRK3 : public TimeIntegrator<T, Solver, SolutionT> {
+ public:
+  SSPRK3(const SolutionT& sol, const Mesh& mesh, T final_time = T(0))
+      : TimeIntegrator<T, Solver, SolutionT>(final_time),
+        tmp1_(sol.clone_shape()),
+        tmp2_(sol.clone_shape()),
+        mesh_(mesh) {}
+
+  // u1 = u + dt L(u); u2 = 3/4 u + 1/4 u1 + dt/4 L(u1); u = 1/3 u + 2/3 u2 + 2dt/3 L(u2),
+  // with L evaluated at t, t + dt and t + dt/2 (SSPRK3.hpp:159-181)
+  void step(Solver& solver, SolutionT& sol, T dt) override {
+    const T t = this->time_;
+    solver.ssprk3_stage(1, dt, nullptr, sol.state(), tmp1_.state(), t);
+    solver.ssprk3_stage(2, dt, sol.state(), tmp1_.state(), tmp2_.state(), t + dt);
+    solver.ssprk3_stage(3, dt, sol.state(), tmp2_.state(), sol.state(), t + T(0.5) * dt);
+    sol.bind(solver.handle());
+  }
+
  private:
   SolutionT tmp1_, tmp2_;
   const Mesh& mesh_;
-  T time_ = T(0), final_time_;
-  int iter_ = 0;
 };
 
-}  // namespace dgsem_hip
+// --- analyzers (analyzer/divergence_checker.hpp:6-27,
+// volume_average_euler.hpp:9-70, analyzer_wrapper.hpp) -------------------------
+// One on-device pass (dgsem_analyze) fills every analyzer of a wrapper:
+// volume, kinetic-energy and entropy integrals with the volume weights
+// w_i w_j (w_k) / |1/J|, and the count of non-finite nodes.
+struct AnalysisResults {
+  double volume = 0, kinetic_energy = 0, entropy = 0, nonfinite = 0, max_speed = 0;
+};
+
+template <class T, std::size_t NVARS>
+struct DivergenceChecker {
+  bool has_nan = false;
+  void init() { has_nan = false; }
+  void take(const AnalysisResults& r) { has_nan = r.nonfinite > 0; }
+};
+
+template <class Equations>
+struct VolumeAverageEuler {
+  using T = typename equations::EquationTraits<Equations>::value_type;
+  T total_volume = T(0), kinetic_energy_integral = T(0), entropy_integral = T(0);
+  void init() { total_volume = kinetic_energy_integral = entropy_integral = T(0); }
+  void take(const AnalysisResults& r) {
+    total_volume = r.volume;
+    kinetic_energy_integral = r.kinetic_energy;
+    entropy_integral = r.entropy;
+  }
+  T mean_kinetic_energy() const { return kinetic_energy_integral / total_volume; }
+  T mean_entropy() const { return entropy_integral / total_volume; }
+};
+
+template <class Basis, class Equations, class... Analyzers>
+struct AnalyzerWrapper {
+  std::tuple<Analyzers...> analyzers;
+  void init() { std::apply([](auto&... a) { (a.init(), ...); }, analyzers); }
+  template <class A>
+  A& get() {
+    return std::get<A>(analyzers);
+  }
+  template <class A>
+  const A& get() const {
+    return std::get<A>(analyzers);
+  }
+  // runs the analysis of sol's state on the GPU and hands the results to
+  // every analyzer (synchronises the solver's stream)
+  template <class SolutionT>
+  void run(const SolutionT& sol) {
+    dgsem_handle* h = sol.handle();
+    if (!h) throw std::runtime_error("analysis of a solution no solver has initialised");
+    double* d = nullptr;
+    if (hipMalloc(&d, 5 * sizeof(double)) != hipSuccess) throw std::bad_alloc();
+    std::unique_ptr<double, void (*)(double*)> guard(d, [](double* p) { (void)hipFree(p); });
+    double out[5];
+    check(dgsem_analyze(h, sol.state(), d), h, "analyze");
+    check(dgsem_synchronize(h), h, "synchronize");
+    if (hipMemcpy(out, d, sizeof(out), hipMemcpyDeviceToHost) != hipSuccess)
+      throw std::runtime_error("hipMemcpy to host failed");
+    AnalysisResults r{out[0], out[1], out[2], out[3], out[4]};
+    init();
+    std::apply([&](auto&... a) { (a.take(r), ...); }, analyzers);
+  }
+};
+
+// --- analysis observers (observer/analysis_observer.hpp:18-186) ---------------
+struct PointwiseAnalysisTag {};
+struct VolumeWeightedAnalysisTag {};
+inline constexpr PointwiseAnalysisTag pointwise_analysis{};
+inline constexpr VolumeWeightedAnalysisTag volume_weighted_analysis{};
+
+template <class Equations>
+struct StopOnNaN {
+  using traits = equations::EquationTraits<Equations>;
+  template <class Analyzer>
+  void operator()(int step, double time, Analyzer& analyzer, bool& has_nan) const {
+    using Checker = DivergenceChecker<typename traits::value_type, traits::NVARS>;
+    if constexpr (requires(Analyzer a) { a.template get<Checker>(); }) {
+      if (analyzer.template get<Checker>().has_nan) {
+        std::cerr << "NaN detected at iter " << step << ", time " << time << std::endl;
+        has_nan = true;
+      }
+    }
+  }
+};
+
+template <class Equations>
+class VolumeAverageCsvWriter {
+ public:
+  explicit VolumeAverageCsvWriter(std::string path) : path_(std::move(path)) {
+    out_ = std::make_shared<std::ofstream>(path_);
+    if (!*out_) {
+      std::cerr << "Failed to open file: " << path_ << std::endl;
+      return;
+    }
+    *out_ << "step,time,volume_average_kinetic_energy,volume_average_entropy\n";
+    *out_ << std::scientific << std::setprecision(16);
+  }
+  template <class Analyzer>
+  void operator()(int step, double time, Analyzer& analyzer, bool&) {
+    if (!*out_) return;
+    const auto& st = analyzer.template get<VolumeAverageEuler<Equations>>();
+    *out_ << step << ',' << time << ',' << st.mean_kinetic_energy() << ',' << st.mean_entropy()
+          << '\n';
+    out_->flush();
+  }
+
+ private:
+  std::string path_;
+  std::shared_ptr<std::ofstream> out_;
+};
+
+template <class Analyzer, class SolutionT, class... Handlers>
+class AnalysisObserver : public SimulationObserver {
+ public:
+  AnalysisObserver(Analyzer& analyzer, const SolutionT& sol, bool analyze_on_finish,
+                   Handlers... handlers)
+      : analyzer_(analyzer), sol_(sol), on_finish_(analyze_on_finish), handlers_(std::move(handlers)...) {}
+  void on_step(int step, double time, bool& has_nan) override { execute(step, time, has_nan); }
+  void on_finish(int step, double time) override {
+    if (!on_finish_) return;
+    bool has_nan = false;
+    execute(step, time, has_nan);
+  }
+
+ private:
+  void execute(int step, double time, bool& has_nan) {
+    analyzer_.run(sol_);
+    std::apply([&](auto&... h) { (h(step, time, analyzer_, has_nan), ...); }, handlers_);
+  }
+  Analyzer& analyzer_;
+  const SolutionT& sol_;
+  bool on_finish_;
+  std::tuple<Handlers...> handlers_;
+};
+
+// make_analysis_observer (analysis_observer.hpp:150-186): pointwise analyses
+// run before every step; volume-weighted ones also once at the end.
+template <class Basis, class Equations, class Analyzer, class SolutionT, class... Handlers>
+std::unique_ptr<SimulationObserver> make_analysis_observer(
+    PointwiseAnalysisTag, Analyzer& analyzer, const SolutionT& sol,
+    const std::array<std::size_t, equations::EquationTraits<Equations>::NDIMS>&,
+    Handlers&&... handlers) {
+  using Obs = AnalysisObserver<Analyzer, SolutionT, std::decay_t<Handlers>...>;
+  return std::make_unique<Obs>(analyzer, sol, false, std::forward<Handlers>(handlers)...);
+}
+template <class Basis, class Equations, class Analyzer, class SolutionT, class ElementContainer,
+          class... Handlers>
+std::unique_ptr<SimulationObserver> make_analysis_observer(
+    VolumeWeightedAnalysisTag, Analyzer& analyzer, const SolutionT& sol, const ElementContainer&,
+    const std::array<std::size_t, equations::EquationTraits<Equations>::NDIMS>&,
+    Handlers&&... handlers) {
+  using Obs = AnalysisObserver<Analyzer, SolutionT, std::decay_t<Handlers>...>;
+  return std::make_unique<Obs>(analyzer, sol, true, std::forward<Handlers>(handlers)...);
+}
+
+// VTUOutputObserver (vtu_output_observer.hpp:18-233): <path>_step<N>.vtu at
+// step 0, every output_interval steps (if positive) and at the end, written
+// from the device state by dgsem_write_vtu (the node coordinates come from
+// the handle; the Coord argument is accepted for interface parity).
+template <class T, class Basis, class SolutionT, class Coord, class Equations>
+class VTUOutputObserver : public SimulationObserver {
+ public:
+  static constexpr std::size_t NDIMS = equations::EquationTraits<Equations>::NDIMS;
+  VTUOutputObserver(const std::string& output_path, const SolutionT& sol, const Coord&,
+                    const std::array<std::size_t, NDIMS>&, int output_interval = -1)
+      : path_(output_path), interval_(output_interval), sol_(sol) {}
+  void on_step(int step, double, bool&) override {
+    if (step == 0) write(step);
+    if (interval_ >= 0 && step % interval_ == 0 && step != last_) write(step);
+  }
+  void on_finish(int step, double) override {
+    if (step != last_) write(step);
+  }
+  std::string make_filename(int step, double) const {
+    return path_ + "_step" + std::to_string(step) + ".vtu";
+  }
+
+ private:
+  void write(int step) {
+    dgsem_handle* h = sol_.handle();
+    if (!h) throw std::runtime_error("VTU output of a solution no solver has initialised");
+    check(dgsem_write_vtu(h, sol_.state(), make_filename(step, 0.0).c_str()), h, "write_vtu");
+    last_ = step;
+  }
+  std::string path_;
+  int interval_, last_ = -1;
+  const SolutionT& sol_;
+};
+
+}  // namespace DGSEM
+
+// The earlier spelling of this header's namespace.
+namespace dgsem_hip = DGSEM;

@@ -218,6 +218,21 @@ class StructuredSolver:
         self._check(self._L.dgsem_get_alpha(self._h, _ptr(a)))
         return a
 
+    def stage_alpha(self, stage: int):
+        """Blending factor the last SSPRK3 stage of that number (1-3)
+        recorded, or the last calc_rhs (0)."""
+        a = torch.empty(self.n_elements, dtype=torch.float64, device="cuda")
+        self._check(self._L.dgsem_get_stage_alpha(self._h, int(stage), _ptr(a)))
+        return a
+
+    def freeze_alpha(self, alpha) -> None:
+        """Blend with this per-element alpha (device tensor, kept referenced
+        here) instead of the indicator's, in the forward evaluations and as
+        the frozen alpha of the derivatives; None returns to the indicator."""
+        self._frozen_alpha = None if alpha is None else alpha.contiguous()
+        ptr = None if alpha is None else _ptr(self._frozen_alpha)
+        self._check(self._L.dgsem_freeze_alpha(self._h, ptr))
+
     def ssprk3_stage(self, stage: int, dt: float, u_base, u_k, u_out) -> None:
         base = _ptr(u_base) if u_base is not None else None
         self._check(self._L.dgsem_ssprk3_stage(self._h, stage, float(dt), base, _ptr(u_k), _ptr(u_out)))

@@ -191,6 +191,21 @@ int dgsem_rhs(dgsem_handle* h, const double* u, double* du, double t);
  * device array of n_elements doubles. */
 int dgsem_get_alpha(const dgsem_handle* h, double* alpha_out);
 
+/* The blending factor each kind of evaluation recorded last, so that a
+ * forward SSPRK3 step leaves the alpha of all three of its stages (the
+ * reference recomputes alpha in every calc_rhs, structured_solver.hpp:
+ * 175-198): stage 1, 2 or 3 -- the last dgsem_ssprk3_stage of that stage
+ * number; stage 0 -- the last dgsem_rhs.  Device array of n_elements. */
+int dgsem_get_stage_alpha(const dgsem_handle* h, int stage, double* alpha_out);
+
+/* Freezes the blending factor (no counterpart in the reference, whose alpha
+ * always comes from the indicator): while `alpha` (device array of
+ * n_elements doubles, read at every launch) is set, dgsem_rhs,
+ * dgsem_ssprk3_stage / _steps blend with it instead of the indicator's
+ * value, and the tangent and adjoint entry points below take it as the
+ * frozen alpha.  NULL returns to the indicator.  Shock capturing only. */
+int dgsem_freeze_alpha(dgsem_handle* h, const double* alpha);
+
 /* One fused SSPRK3 stage: RHS of u_k followed by the stage combination of
  * parallel_ma2 / parallel_ma3 (SSPRK3.hpp:12-143) as SSPRK3::step uses them
  * (SSPRK3.hpp:159-181):
@@ -212,10 +227,16 @@ int dgsem_synchronize(dgsem_handle* h);
 
 /* ---------------------------------------------------------------------
  * Tangent and discrete adjoint.  The reference has no derivatives; these
- * differentiate the right-hand side this library computes (split form,
- * LLF surface flux, Euler or Navier-Stokes with BR1; the shock-capturing
- * blend and the positivity floor are not differentiated) exactly, up to
- * rounding, with J = dL/du at the state u:
+ * differentiate the right-hand side this library computes exactly, up to
+ * rounding, with J = dL/du at the state u: split form or shock capturing
+ * (the blend (1 - alpha) split form + alpha FV with the LLF subcell flux,
+ * volume_flux.hpp:449-630, :821-869), LLF surface flux, Euler or
+ * Navier-Stokes with BR1, periodic meshes.  The blending factor is not
+ * differentiated: it is held at the value the forward evaluation at the same
+ * state computes (the indicator at u; at each stage input for the step
+ * functions), or at the array set with dgsem_freeze_alpha.  The HLLC fluxes,
+ * the positivity floor and non-periodic faces are not differentiated
+ * (DGSEM_E_UNSUPPORTED).
  * --------------------------------------------------------------------- */
 /* out = J(u) du */
 int dgsem_rhs_jvp(dgsem_handle* h, const double* u, const double* du, double* out);

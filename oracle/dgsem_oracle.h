@@ -110,9 +110,12 @@ void orc_gradients(orc_case* c, const double* u, double* gradient);
 void orc_set_boundary(orc_case* c, int face, int kind, const double* state,
                       const double* values);
 
-/* SSPRK3::step (SSPRK3.hpp:159-181), nsteps times with fixed dt. */
+/* test hook: blend with a fixed alpha [E] (kept by reference) instead of the
+ * indicator's; NULL restores the indicator */
+void orc_set_alpha_override(orc_case* c, const double* alpha);
 /* the positivity floor of a stage update, applied to u in place */
 void orc_apply_floor(const orc_case* c, double* u);
+/* SSPRK3::step (SSPRK3.hpp:159-181), nsteps times with fixed dt. */
 void orc_ssprk3(orc_case* c, double* u, double dt, int nsteps);
 
 /* Pointwise kernels exposed for unit tests. */
@@ -126,6 +129,11 @@ void orc_flux_hllc(int dim, double gamma, const double* ul, const double* ur,
                    const double* normal, double* f);
 void orc_physical_flux(int dim, double gamma, const double* u,
                        const double* normal, double* f);
+
+/* gradient_variables and viscous_flux (compressible_navier_stokes{2D,3D}.hpp)
+ * with the case's gamma, mu and kappa; grad is [dim][nvars] */
+void orc_gradient_variables(const orc_case* c, const double* u, double* q);
+void orc_viscous_flux(const orc_case* c, const double* q, const double* grad, int dim, double* f);
 
 /* Volume-weighted kinetic energy sum (volume_average_euler.hpp:30-53 with the
  * weights of volume_weighted_analyzer_functor.hpp:108-134). */

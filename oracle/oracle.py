@@ -74,6 +74,9 @@ def lib():
     L.orc_gradients.argtypes = [vp, _dp, _dp]
     L.orc_ssprk3.argtypes = [vp, _dp, C.c_double, C.c_int]
     L.orc_apply_floor.argtypes = [vp, _dp]
+    L.orc_set_alpha_override.argtypes = [vp, vp]
+    L.orc_gradient_variables.argtypes = [vp, _dp, _dp]
+    L.orc_viscous_flux.argtypes = [vp, _dp, _dp, C.c_int, _dp]
     L.orc_ln_mean.restype = C.c_double
     L.orc_ln_mean.argtypes = [C.c_double, C.c_double]
     for name in ("orc_flux_chandrashekar", "orc_flux_llf", "orc_flux_hllc"):
@@ -276,6 +279,24 @@ class Case:
         u = np.array(u, dtype=np.float64, order="C", copy=True)
         lib().orc_ssprk3(self._h, u, float(dt), int(nsteps))
         return u
+
+    def gradient_variables(self, u):
+        q = np.zeros(self.nvars)
+        lib().orc_gradient_variables(self._h, np.ascontiguousarray(u, dtype=np.float64), q)
+        return q
+
+    def viscous_flux(self, q, grad, dim):
+        """grad: [dim][nvars] gradient of the gradient variables."""
+        f = np.zeros(self.nvars)
+        lib().orc_viscous_flux(self._h, np.ascontiguousarray(q, dtype=np.float64),
+                               np.ascontiguousarray(grad, dtype=np.float64), int(dim), f)
+        return f
+
+    def set_alpha_override(self, alpha):
+        """Blend every later shock-capturing RHS with this alpha [E] instead
+        of the indicator's (None restores the indicator)."""
+        self._alpha_keep = None if alpha is None else np.ascontiguousarray(alpha, dtype=np.float64)
+        lib().orc_set_alpha_override(self._h, None if alpha is None else self._alpha_keep.ctypes.data)
 
     def apply_floor(self, u):
         """The positivity floor of a stage update on a copy of u."""

@@ -1,0 +1,212 @@
+"""Tangent and adjoint of the shock-capturing right-hand side with the
+blending factor alpha frozen: the blend (1 - alpha) split form + alpha FV
+with the LLF subcell flux (volume_flux.hpp:449-630, :821-869), on the
+Kelvin-Helmholtz (2D) and Sedov-with-LLF-subcells (3D) configurations, each
+with troubled and untroubled elements.
+
+Checks:
+  * the forward records each stage's alpha (dgsem_get_stage_alpha) and a
+    frozen alpha (dgsem_freeze_alpha) reproduces the indicator's RHS bit for
+    bit;
+  * tangent against central finite differences (h = 1e-7) of the oracle's
+    RHS with alpha overridden to the same frozen values (relative 1e-6);
+  * adjoint against tangent: <w, J v> = <J^T w, v> (relative 1e-12), for the
+    RHS and for one SSPRK3 step;
+  * the checkpointed gradient of the time-integrated kinetic energy of a
+    Kelvin-Helmholtz run against central finite differences of the GPU
+    forward with every stage's alpha frozen at the unperturbed run's values
+    (relative 1e-6).
+
+The 2D states are the Kelvin-Helmholtz state with a smooth perturbation of
+density, velocity and pressure.  The plain state is symmetric: along x its
+x-velocity and sound speed are constant, so the LLF wave speed's two max()
+arguments tie on every x-direction subcell face.  The sound speeds tie only
+up to the rounding of the recomputed pressure, a kink some 1e-16 away, which
+a central difference straddles and a derivative cannot see.  The
+perturbation moves every such kink far from the state.  The finite
+differences use h = 1e-7, which keeps the O(h) error of an exact tie
+(equal traces on element faces) below 2e-7.
+"""
+import numpy as np
+import pytest
+import torch
+
+import dgsem_amd as dg
+from dgsem_amd import adjoint, ensemble
+from gpu_pairs import make_pair
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # dim, N, cells, alpha_max, geometry
+    (2, 3, (8, 8), 0.002, dg.GEOMETRY_NODAL),
+    (2, 3, (8, 8), 0.5, dg.GEOMETRY_AFFINE),
+    (2, 4, (6, 5), 0.5, dg.GEOMETRY_NODAL),
+    (3, 3, (4, 4, 4), 0.5, dg.GEOMETRY_NODAL),
+    (3, 2, (4, 4, 4), 0.5, dg.GEOMETRY_AFFINE),
+]
+
+
+def ids(c):
+    return f"{c[0]}d-N{c[1]}-amax{c[3]}-{'nodal' if c[4] == dg.GEOMETRY_NODAL else 'affine'}"
+
+
+def perturbed_khi(x, gamma=1.4):
+    """The Kelvin-Helmholtz state (reference layout) with smooth
+    perturbations of density, velocity and pressure; x: [e][dof][2]."""
+    X, Y = x[..., 0], x[..., 1]
+    B = np.tanh(15 * Y + 7.5) - np.tanh(15 * Y - 7.5)
+    rho = (0.5 + 0.75 * B) * (1 + 0.05 * np.sin(2 * np.pi * X + 0.7) * np.sin(np.pi * Y + 0.3))
+    vx = 0.5 * (B - 1) + 0.03 * np.cos(np.pi * X + 0.4 * Y)
+    vy = 0.1 * np.sin(2 * np.pi * X) + 0.02 * np.sin(np.pi * Y + 0.9)
+    p = 1.0 + 0.05 * np.cos(2 * np.pi * X + 1.1) * np.sin(2 * np.pi * Y + 0.2)
+    E = p / (gamma - 1) + 0.5 * rho * (vx * vx + vy * vy)
+    return np.stack([rho, rho * vx, rho * vy, E], axis=-1)
+
+
+def pair(case):
+    dim, N, cells, amax, geo = case
+    box = ([-1.0] * dim, [1.0] * dim)
+    oc, gs = make_pair(dim, N, cells, *box, geometry=geo, volume=dg.SHOCK_CAPTURING, amax=amax,
+                       amin=0.001)
+    u = perturbed_khi(oc.coordinates()) if dim == 2 else oc.init_sedov()
+    return oc, gs, u
+
+
+def direction(u, seed):
+    """A random direction scaled per variable like the state."""
+    rng = np.random.default_rng(seed)
+    scale = np.mean(np.abs(u), axis=(0, 1)) + 1e-3
+    return rng.standard_normal(u.shape) * (np.abs(u) + scale)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("case", CASES, ids=[ids(c) for c in CASES])
+def test_frozen_alpha_reproduces_the_indicator(gpu, case):
+    oc, gs, u0 = pair(case)
+    u = gs.from_reference_layout(u0)
+    du, du2 = gs.empty(), gs.empty()
+    gs.calc_rhs(u, du)
+    alpha = gs.stage_alpha(0)
+    assert torch.equal(alpha, gs.alpha())
+    troubled = int((alpha > 0).sum())
+    print(f"case {ids(case)}: {troubled} of {gs.n_elements} elements troubled", flush=True)
+    assert 0 < troubled < gs.n_elements
+    gs.freeze_alpha(alpha)
+    gs.calc_rhs(u, du2)
+    gs.freeze_alpha(None)
+    assert torch.equal(du, du2)
+    # the three stages of a step each keep their alpha
+    t1, t2, t3 = gs.empty(), gs.empty(), gs.empty()
+    gs.ssprk3_stage(1, 1e-4, None, u, t1)
+    gs.ssprk3_stage(2, 1e-4, u, t1, t2)
+    gs.ssprk3_stage(3, 1e-4, u, t2, t3)
+    for k, x in ((1, u), (2, t1), (3, t2)):
+        gs.calc_rhs(x, du)
+        assert torch.equal(gs.stage_alpha(k), gs.stage_alpha(0)), k
+    assert torch.equal(gs.alpha(), gs.stage_alpha(0))
+
+
+@pytest.mark.parametrize("case", CASES, ids=[ids(c) for c in CASES])
+def test_tangent_matches_oracle_finite_differences(gpu, case):
+    oc, gs, u0 = pair(case)
+    _, alpha = oc.rhs(u0, with_alpha=True)
+    oc.set_alpha_override(alpha)
+    v = direction(u0, 4)
+    h = 1e-7
+    fd = (oc.rhs(u0 + h * v) - oc.rhs(u0 - h * v)) / (2 * h)
+    oc.set_alpha_override(None)
+    u, vt = gs.from_reference_layout(u0), gs.from_reference_layout(v)
+    jv = gs.to_reference_layout(gs.rhs_jvp(u, vt))
+    err = rel(jv, fd)
+    print(f"case jvp fd {ids(case)}: rel {err:.2e}", flush=True)
+    assert err <= 1e-6
+
+
+@pytest.mark.parametrize("case", CASES, ids=[ids(c) for c in CASES])
+def test_adjoint_dot_product(gpu, case):
+    _, gs, u0 = pair(case)
+    u = gs.from_reference_layout(u0)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    v = torch.randn(gs.shape, dtype=torch.float64, device="cuda", generator=g)
+    w = torch.randn(gs.shape, dtype=torch.float64, device="cuda", generator=g)
+    jv = gs.rhs_jvp(u, v)
+    lhs = torch.sum(w * jv).item()
+    rhs = torch.sum(gs.rhs_vjp(u, w) * v).item()
+    scale = torch.sum(torch.abs(w * jv)).item()
+    print(f"case vjp dot {ids(case)}: {abs(lhs - rhs) / scale:.2e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("case", [CASES[1], CASES[3]], ids=[ids(CASES[1]), ids(CASES[3])])
+def test_step_adjoint_dot_product(gpu, case):
+    _, gs, u0 = pair(case)
+    u = gs.from_reference_layout(u0)
+    dt = 1e-4
+    u1, u2 = gs.ssprk3_stages(u, dt)
+    g = torch.Generator(device="cuda").manual_seed(11)
+    v = torch.randn(gs.shape, dtype=torch.float64, device="cuda", generator=g)
+    w = torch.randn(gs.shape, dtype=torch.float64, device="cuda", generator=g)
+    jv = gs.step_jvp(dt, u, u1, u2, v)
+    jtw = gs.step_vjp(dt, u, u1, u2, w)
+    lhs, rhs = torch.sum(w * jv).item(), torch.sum(jtw * v).item()
+    assert abs(lhs - rhs) <= 1e-12 * torch.sum(torch.abs(w * jv)).item()
+
+
+def frozen_objective(gs, u0, dt, nsteps, alphas):
+    """J = sum_{n < nsteps} dt KE(u_n) of a forward run whose stages blend
+    with the given alphas (alphas[n][k] for stage k + 1 of step n)."""
+    ke = torch.empty(nsteps, dtype=torch.float64, device="cuda")
+    u, u1, u2, nxt = u0.clone(), gs.empty(), gs.empty(), gs.empty()
+    for n in range(nsteps):
+        gs.kinetic_energy(u, ke[n:n + 1])
+        gs.freeze_alpha(alphas[n][0])
+        gs.ssprk3_stage(1, dt, None, u, u1)
+        gs.freeze_alpha(alphas[n][1])
+        gs.ssprk3_stage(2, dt, u, u1, u2)
+        gs.freeze_alpha(alphas[n][2])
+        gs.ssprk3_stage(3, dt, u, u2, nxt)
+        u, nxt = nxt, u
+    gs.freeze_alpha(None)
+    return dt * float(np.sum(ke.cpu().numpy()))
+
+
+@pytest.mark.parametrize("amax", [0.002, 0.5])
+def test_checkpointed_khi_gradient_matches_frozen_finite_differences(gpu, amax):
+    """Kelvin-Helmholtz 8^2 (perturbed), N = 3, 12 steps at the CFL-0.5
+    step, checkpoints every 5 steps."""
+    N, cells, nsteps, every = 3, (8, 8), 12, 5
+    cfg = dg.SolverConfig(dim=2, polydeg=N, cells=cells, domain_left=(-1, -1), domain_right=(1, 1),
+                          volume_integral=dg.SHOCK_CAPTURING, alpha_max=amax, alpha_min=0.001)
+    gs = dg.StructuredSolver(cfg)
+    x = gs.node_coordinates()
+    ref = perturbed_khi(x)
+    dt = ensemble.khi_dt(ref, N, cells)
+    u0 = gs.from_reference_layout(ref)
+    # the unperturbed run's alpha of every stage
+    alphas = []
+    u, u1, u2, nxt = u0.clone(), gs.empty(), gs.empty(), gs.empty()
+    for _ in range(nsteps):
+        gs.ssprk3_stage(1, dt, None, u, u1)
+        gs.ssprk3_stage(2, dt, u, u1, u2)
+        gs.ssprk3_stage(3, dt, u, u2, nxt)
+        alphas.append([gs.stage_alpha(k).clone() for k in (1, 2, 3)])
+        u, nxt = nxt, u
+    assert any(bool((a > 0).any()) for step in alphas for a in step)
+    J, grad = adjoint.kinetic_energy_objective(gs, u0, dt, nsteps, every)
+    J0 = frozen_objective(gs, u0, dt, nsteps, alphas)
+    assert abs(J - J0) <= 1e-14 * abs(J)
+    g = torch.Generator(device="cuda").manual_seed(13)
+    worst = 0.0
+    for _ in range(3):
+        d = torch.randn(gs.shape, dtype=torch.float64, device="cuda", generator=g) * u0.abs()
+        h = 1e-7
+        fd = (frozen_objective(gs, (u0 + h * d).contiguous(), dt, nsteps, alphas)
+              - frozen_objective(gs, (u0 - h * d).contiguous(), dt, nsteps, alphas)) / (2 * h)
+        ad = torch.sum(grad * d).item()
+        worst = max(worst, abs(fd - ad) / abs(ad))
+    print(f"case khi objective amax={amax}: J {J:.6e}, worst rel {worst:.2e}", flush=True)
+    assert worst <= 1e-6

@@ -113,3 +113,49 @@ def test_cpp_viscous_shock_tube_matches_oracle(gpu, tmp_path):
     ref = c.ssprk3(u0.reshape(c.shape), dt, steps)
     got = got.reshape(ref.shape)
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-10
+
+
+@pytest.mark.gpu
+def test_cpp_observers_example(gpu, tmp_path):
+    """The reference-style driver with observers (taylor_green_observers):
+    PrintObserver lines every 5 steps, VTU files at steps 0, 10 and 20, the
+    NaN check and the volume-average CSV before every step and at the end,
+    whose kinetic energies match the Python path's on-device analysis of the
+    same run."""
+    import torch
+
+    import dgsem_amd as dg
+    from dgsem_amd import tgv
+
+    exe = os.path.join(ROOT, "dgsem_amd", "examples", "taylor_green_observers")
+    assert os.path.exists(exe), "build the library first (dgsem_amd/build.py)"
+    cells, N, steps = 4, 3, 20
+    prefix = str(tmp_path / "tgv")
+    print("case cpp observers tgv 4^3 N=3 20 steps", flush=True)
+    r = subprocess.run([exe, str(cells), str(N), str(steps), prefix], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    assert [l.split()[1] for l in lines if l.startswith("Iter:")] == ["0", "5", "10", "15"]
+    assert any(l.startswith(f"Finished at iter {steps}") for l in lines)
+    dt = float([l for l in lines if l.startswith("dt ")][0].split()[1])
+    for k in (0, 10, 20):
+        assert os.path.getsize(f"{prefix}_step{k}.vtu") > 0
+    rows = open(prefix + "_averages.csv").read().splitlines()
+    assert rows[0] == "step,time,volume_average_kinetic_energy,volume_average_entropy"
+    data = np.array([[float(x) for x in row.split(",")] for row in rows[1:]])
+    assert data.shape == (steps + 1, 4)  # before every step, and once at the end
+    # the same run through the Python interface
+    s = dg.StructuredSolver(dg.SolverConfig(dim=3, polydeg=N, cells=(cells,) * 3,
+                                            domain_left=(0, 0, 0),
+                                            domain_right=(2 * math.pi,) * 3))
+    u = s.from_reference_layout(tgv.tgv_state(s.node_coordinates()))
+    rk = dg.SSPRK3(s)
+    ke = []
+    for n in range(steps + 1):
+        ke.append(s.analyze(u)["mean_kinetic_energy"])
+        if n < steps:
+            rk.step(u, dt)
+    torch.cuda.synchronize()
+    assert np.allclose(data[:, 2], ke, rtol=1e-13, atol=0.0)
+    assert np.all(np.diff(data[:, 0]) == 1) and np.isfinite(data).all()
