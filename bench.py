@@ -48,14 +48,27 @@ def volume_flops(E: int, N: int) -> float:
     return float(E) * (pairs * C_PAIR_3D + n ** 3 * C_NODE_3D)
 
 
-def stage_bytes(E: int, N: int, geometry: str) -> float:
+def stage_bytes(E: int, N: int, geometry: str, box_cells=None) -> float:
     """Algorithmic HBM bytes of one fused RHS + SSPRK3 stage launch, averaged
     over the three stage kinds: the state read once (stage 1), the state and
     the base state (stages 2 and 3), the result written once, and the metric
-    terms (nodal: Ja and 1/J per node; affine: 10 doubles per element)."""
+    terms (nodal: Ja and 1/J per node; affine: 10 doubles per element).
+
+    box_cells: the cells per direction of a periodic box mesh whose nodal
+    metric terms the line kernel reads from the box table.  Ja^r_c depends
+    only on the element's two indices other than c, so the table holds, per
+    direction c, one row of 3 x (N+1)^3 doubles for each pair of those
+    indices; with 1/J still per node."""
     nodes = E * (N + 1) ** 3
     state = nodes * 5 * 8
-    metrics = nodes * (3 * 3 + 1) * 8 if geometry == "nodal" else E * 10 * 8
+    if geometry != "nodal":
+        metrics = E * 10 * 8
+    elif box_cells is not None:
+        c = box_cells
+        pairs = c[1] * c[2] + c[2] * c[0] + c[0] * c[1]
+        metrics = pairs * 3 * (N + 1) ** 3 * 8 + nodes * 8
+    else:
+        metrics = nodes * (3 * 3 + 1) * 8
     return state * (1.0 + 2.0 / 3.0) + state + metrics
 
 
@@ -438,7 +451,10 @@ def main() -> int:
     t_kernel_ms = ev0.elapsed_time(ev1) / (3 * args.steps)
 
     flops = volume_flops(E, N)
-    hbm_bytes = stage_bytes(E, N, args.geometry)
+    # the line kernels (N < 7) read a box mesh's nodal metric terms from the
+    # box table; the node kernel (N = 7) reads the per-node arrays
+    hbm_bytes = stage_bytes(E, N, args.geometry,
+                            (cells,) * 3 if args.geometry == "nodal" and N < 7 else None)
     intensity = flops / hbm_bytes
     ridge = FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
     bound = "hbm" if intensity < ridge else "fp64"

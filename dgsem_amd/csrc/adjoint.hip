@@ -200,10 +200,10 @@ int alloc_once(dgsem_handle* h, double** p, size_t count) {
 int ad_prepare(dgsem_handle* h) {
   const auto& c = h->cfg;
   const bool sc = c.volume_integral == DGSEM_VOLUME_SHOCK_CAPTURING;
-  if (sc && c.fv_flux != DGSEM_FLUX_LLF)
-    return afail(h, DGSEM_E_UNSUPPORTED, "tangent and adjoint are built for the LLF subcell flux");
-  if (c.surface_flux != DGSEM_FLUX_LLF)
-    return afail(h, DGSEM_E_UNSUPPORTED, "tangent and adjoint are built for the LLF surface flux");
+  if (c.equation == DGSEM_NAVIER_STOKES &&
+      (c.surface_flux != DGSEM_FLUX_LLF || (sc && c.fv_flux != DGSEM_FLUX_LLF)))
+    return afail(h, DGSEM_E_UNSUPPORTED,
+                 "the Navier-Stokes tangent and adjoint are built for the LLF fluxes");
   if (c.positivity_floor > 0.0)
     return afail(h, DGSEM_E_UNSUPPORTED, "the positivity floor is not differentiated");
   for (int f = 0; f < 2 * c.dim; ++f)

@@ -22,6 +22,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.hpp"
 #include "parabolic_impl.hpp"
 #include "physics_ad.hpp"
@@ -246,7 +248,7 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) jvp_visc_kernel(
   }
 }
 
-template <int DIM, int N, int EPW, bool NODAL, bool NS, bool SC>
+template <int DIM, int N, int EPW, bool NODAL, bool NS, bool SC, int SF, int FF>
 __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) jvp_rhs_kernel(const AdParams a) {
   using S = Shape<DIM, N>;
   constexpr int n = S::n, NV = S::NV, NN = S::NN, FN = S::FN, L = S::LINES, NP = S::NP;
@@ -352,8 +354,11 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) jvp_rhs_kernel(c
           load_state<DIM, NN>(a.v, e, dofs[fi - 1], dul);
           load_state<DIM, NN>(a.v, e, dofs[fi], dur);
           subcell_normal<DIM, N, NODAL>(p, e, dir, line, fi - 1, nrm);
-          llf_jvp<DIM>(p.gamma, ul, node_prims<DIM>(p.gamma, ul), ur, node_prims<DIM>(p.gamma, ur), dul,
-                       dur, nrm, df);
+          if constexpr (FF == kFluxHLLC)
+            hllc_jvp<DIM>(p.gamma, ul, ur, dul, dur, nrm, df);
+          else
+            llf_jvp<DIM>(p.gamma, ul, node_prims<DIM>(p.gamma, ul), ur, node_prims<DIM>(p.gamma, ur),
+                         dul, dur, nrm, df);
           const double wl = alpha * p.inv_w[fi - 1], wr = alpha * p.inv_w[fi];
           for (int v = 0; v < NV; ++v) {
             acc[fi - 1][v] += wl * df[v];
@@ -380,8 +385,11 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) jvp_rhs_kernel(c
       ja(er, dr, nrm);
       const double sg = sign_j<NODAL, DIM, NN>(p, er, dr);
       for (int c = 0; c < DIM; ++c) nrm[c] *= sg;
-      llf_jvp<DIM>(p.gamma, ul, node_prims<DIM>(p.gamma, ul), ur, node_prims<DIM>(p.gamma, ur), dul,
-                   dur, nrm, df);
+      if constexpr (SF == kFluxHLLC)
+        hllc_jvp<DIM>(p.gamma, ul, ur, dul, dur, nrm, df);
+      else
+        llf_jvp<DIM>(p.gamma, ul, node_prims<DIM>(p.gamma, ul), ur, node_prims<DIM>(p.gamma, ur), dul,
+                     dur, nrm, df);
       if (lower) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[0][v] -= df[v] * p.lhat0;
@@ -573,7 +581,7 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_visc2_kernel
   }
 }
 
-template <int DIM, int N, int EPW, bool NODAL, bool NS, bool SC>
+template <int DIM, int N, int EPW, bool NODAL, bool NS, bool SC, int SF, int FF>
 __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(const AdParams a) {
   using S = Shape<DIM, N>;
   constexpr int n = S::n, NV = S::NV, NN = S::NN, FN = S::FN, L = S::LINES, NP = S::NP;
@@ -677,8 +685,12 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(c
       for (int v = 0; v < NV; ++v) g[v] = p.lhatN * rl[v] - p.lhat0 * rrv[v];
       double ubf[NV];
       for (int v = 0; v < NV; ++v) ubf[v] = 0.0;
-      llf_vjp_side<DIM>(p.gamma, ul, node_prims<DIM>(p.gamma, ul), ur, node_prims<DIM>(p.gamma, ur),
-                        nrm, g, !lower, ubf);
+      // this element is the face's right state at its lower face
+      if constexpr (SF == kFluxHLLC)
+        hllc_vjp<DIM>(p.gamma, ul, ur, nrm, g, !lower, lower, ubf, ubf);
+      else
+        llf_vjp_side<DIM>(p.gamma, ul, node_prims<DIM>(p.gamma, ul), ur, node_prims<DIM>(p.gamma, ur),
+                          nrm, g, !lower, ubf);
       if (lower) {
         for (int v = 0; v < NV; ++v) ub_lo[v] = ubf[v];
       } else {
@@ -724,9 +736,13 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(c
               g[v] = wl * rr[v * NN + dofs[fi - 1]] - wr * rr[v * NN + dofs[fi]];
               ubl[v] = ubr[v] = 0.0;
             }
-            const NodePrims<DIM> ql = node_prims<DIM>(p.gamma, ul), qr = node_prims<DIM>(p.gamma, ur);
-            llf_vjp_side<DIM>(p.gamma, ul, ql, ur, qr, nrm, g, true, ubl);
-            llf_vjp_side<DIM>(p.gamma, ul, ql, ur, qr, nrm, g, false, ubr);
+            if constexpr (FF == kFluxHLLC) {
+              hllc_vjp<DIM>(p.gamma, ul, ur, nrm, g, true, true, ubl, ubr);
+            } else {
+              const NodePrims<DIM> ql = node_prims<DIM>(p.gamma, ul), qr = node_prims<DIM>(p.gamma, ur);
+              llf_vjp_side<DIM>(p.gamma, ul, ql, ur, qr, nrm, g, true, ubl);
+              llf_vjp_side<DIM>(p.gamma, ul, ql, ur, qr, nrm, g, false, ubr);
+            }
             for (int v = 0; v < NV; ++v) {
               s_ub[(slot * NV + v) * NN + dofs[fi - 1]] += ubl[v];
               s_ub[(slot * NV + v) * NN + dofs[fi]] += ubr[v];
@@ -767,40 +783,58 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(c
   }
 }
 
-template <int DIM, int N, bool NODAL, bool SC>
-hipError_t launch_jvp_t(bool ns, const AdParams& a, hipStream_t s) {
+// Navier-Stokes handles take the LLF fluxes only (the ABI checks it), so
+// their kernels are built for LLF alone.
+template <int DIM, int N, bool NODAL, bool SC, bool ADJ>
+hipError_t launch_ad_t(bool ns, const AdParams& a, hipStream_t s) {
   constexpr int EPW = ad_elems_per_block<DIM, N>();
   constexpr int BLOCK = EPW * Shape<DIM, N>::LINES;
   const int64_t nb = (a.k.E + EPW - 1) / EPW;
   if (nb <= 0) return hipSuccess;
+  const dim3 grid((unsigned)nb), block(BLOCK);
   if (ns) {
-    hipLaunchKernelGGL((jvp_grad_kernel<DIM, N, EPW, NODAL>), dim3((unsigned)nb), dim3(BLOCK), 0, s, a);
-    hipLaunchKernelGGL((jvp_visc_kernel<DIM, N, EPW, NODAL>), dim3((unsigned)nb), dim3(BLOCK), 0, s, a);
-    hipLaunchKernelGGL((jvp_rhs_kernel<DIM, N, EPW, NODAL, true, SC>), dim3((unsigned)nb), dim3(BLOCK),
-                       0, s, a);
-  } else {
-    hipLaunchKernelGGL((jvp_rhs_kernel<DIM, N, EPW, NODAL, false, SC>), dim3((unsigned)nb),
-                       dim3(BLOCK), 0, s, a);
+    if constexpr (ADJ) {
+      hipLaunchKernelGGL((vjp_visc1_kernel<DIM, N, EPW, NODAL>), grid, block, 0, s, a);
+      hipLaunchKernelGGL((vjp_visc2_kernel<DIM, N, EPW, NODAL>), grid, block, 0, s, a);
+      hipLaunchKernelGGL((vjp_rhs_kernel<DIM, N, EPW, NODAL, true, SC, kFluxLLF, kFluxLLF>), grid,
+                         block, 0, s, a);
+    } else {
+      hipLaunchKernelGGL((jvp_grad_kernel<DIM, N, EPW, NODAL>), grid, block, 0, s, a);
+      hipLaunchKernelGGL((jvp_visc_kernel<DIM, N, EPW, NODAL>), grid, block, 0, s, a);
+      hipLaunchKernelGGL((jvp_rhs_kernel<DIM, N, EPW, NODAL, true, SC, kFluxLLF, kFluxLLF>), grid,
+                         block, 0, s, a);
+    }
+    return hipGetLastError();
   }
-  return hipGetLastError();
+  const bool shllc = a.k.surface_flux == kFluxHLLC, fhllc = SC && a.k.fv_flux == kFluxHLLC;
+  auto go = [&](auto sf, auto ff) {
+    constexpr int SFv = decltype(sf)::value, FFv = decltype(ff)::value;
+    if constexpr (ADJ)
+      hipLaunchKernelGGL((vjp_rhs_kernel<DIM, N, EPW, NODAL, false, SC, SFv, FFv>), grid, block, 0, s,
+                         a);
+    else
+      hipLaunchKernelGGL((jvp_rhs_kernel<DIM, N, EPW, NODAL, false, SC, SFv, FFv>), grid, block, 0, s,
+                         a);
+    return hipGetLastError();
+  };
+  using LLF = std::integral_constant<int, kFluxLLF>;
+  using HLLC = std::integral_constant<int, kFluxHLLC>;
+  if constexpr (SC) {
+    if (shllc) return fhllc ? go(HLLC{}, HLLC{}) : go(HLLC{}, LLF{});
+    return fhllc ? go(LLF{}, HLLC{}) : go(LLF{}, LLF{});
+  } else {
+    return shllc ? go(HLLC{}, LLF{}) : go(LLF{}, LLF{});
+  }
+}
+
+template <int DIM, int N, bool NODAL, bool SC>
+hipError_t launch_jvp_t(bool ns, const AdParams& a, hipStream_t s) {
+  return launch_ad_t<DIM, N, NODAL, SC, false>(ns, a, s);
 }
 
 template <int DIM, int N, bool NODAL, bool SC>
 hipError_t launch_vjp_t(bool ns, const AdParams& a, hipStream_t s) {
-  constexpr int EPW = ad_elems_per_block<DIM, N>();
-  constexpr int BLOCK = EPW * Shape<DIM, N>::LINES;
-  const int64_t nb = (a.k.E + EPW - 1) / EPW;
-  if (nb <= 0) return hipSuccess;
-  if (ns) {
-    hipLaunchKernelGGL((vjp_visc1_kernel<DIM, N, EPW, NODAL>), dim3((unsigned)nb), dim3(BLOCK), 0, s, a);
-    hipLaunchKernelGGL((vjp_visc2_kernel<DIM, N, EPW, NODAL>), dim3((unsigned)nb), dim3(BLOCK), 0, s, a);
-    hipLaunchKernelGGL((vjp_rhs_kernel<DIM, N, EPW, NODAL, true, SC>), dim3((unsigned)nb), dim3(BLOCK),
-                       0, s, a);
-  } else {
-    hipLaunchKernelGGL((vjp_rhs_kernel<DIM, N, EPW, NODAL, false, SC>), dim3((unsigned)nb),
-                       dim3(BLOCK), 0, s, a);
-  }
-  return hipGetLastError();
+  return launch_ad_t<DIM, N, NODAL, SC, true>(ns, a, s);
 }
 
 template <int DIM, int N, bool NODAL>

@@ -31,6 +31,15 @@ enum FluxKind : int { kFluxLLF = 0, kFluxHLLC = 1 };
 
 DG_HD double dmax(double a, double b) { return (a < b) ? b : a; }
 DG_HD double dmin(double a, double b) { return (b < a) ? b : a; }
+// the same for other scalar types (the derivative code's dual numbers)
+template <class T>
+DG_HD T dmax(T a, T b) {
+  return (a < b) ? b : a;
+}
+template <class T>
+DG_HD T dmin(T a, T b) {
+  return (b < a) ? b : a;
+}
 
 // 1 / x: on the GPU the hardware reciprocal refined by two Newton steps,
 // which is within an ulp of the IEEE quotient for half the instructions of
@@ -72,20 +81,21 @@ DG_HD double pressure(double gamma, const double* u) {
 }
 
 // Physical flux F(u) . n: compressible_euler2D.hpp:25-57,
-// compressible_euler3D.hpp:26-67 (sum of the per-direction fluxes).
-template <int DIM>
-DG_HD void physical_flux_n(double gamma, const double* u, const double* n, double* f) {
+// compressible_euler3D.hpp:26-67 (sum of the per-direction fluxes).  T is
+// double, or the dual numbers of the derivative code (physics_ad.hpp).
+template <int DIM, class T = double>
+DG_HD void physical_flux_n(double gamma, const T* u, const double* n, T* f) {
   constexpr int NV = DIM + 2;
-  const double rho = u[0], rhoE = u[NV - 1];
-  double v[DIM];
-  double v2 = 0.0;
+  const T rho = u[0], rhoE = u[NV - 1];
+  T v[DIM];
+  T v2 = 0.0;
   for (int k = 0; k < DIM; ++k) v[k] = u[k + 1] / rho;
   for (int k = 0; k < DIM; ++k) v2 += v[k] * v[k];
-  const double p = (gamma - 1.0) * (rhoE - 0.5 * rho * v2);
+  const T p = (gamma - 1.0) * (rhoE - 0.5 * rho * v2);
   for (int i = 0; i < NV; ++i) f[i] = 0.0;
   for (int d = 0; d < DIM; ++d) {
     // flux in direction d: (m_d, m_d v + p e_d, v_d (E + p))
-    double fd[NV];
+    T fd[NV];
     fd[0] = u[d + 1];
     for (int k = 0; k < DIM; ++k) fd[k + 1] = u[d + 1] * v[k] + (k == d ? p : 0.0);
     fd[NV - 1] = v[d] * (rhoE + p);
@@ -123,15 +133,16 @@ DG_HD void flux_llf(double gamma, const double* ul, const double* ur, const doub
 // (Roe-averaged wave-speed bounds, contact speed, star states) in the frame of
 // the unit normal with the tangential velocity carried through, and scales
 // the result by |n|.
-template <int DIM>
-DG_HD void flux_hllc(double gamma, const double* ul, const double* ur, const double* n, double* f) {
+template <int DIM, class T = double>
+DG_HD void flux_hllc(double gamma, const T* ul, const T* ur, const double* n, T* f) {
   constexpr int NV = DIM + 2;
-  double nrm = 0.0, nh[DIM], vl[DIM], vr[DIM];
+  double nrm = 0.0, nh[DIM];
+  T vl[DIM], vr[DIM];
   for (int d = 0; d < DIM; ++d) nrm += n[d] * n[d];
   nrm = sqrt(nrm);
   for (int d = 0; d < DIM; ++d) nh[d] = n[d] / nrm;
-  const double rl = ul[0], rr = ur[0], El = ul[NV - 1], Er = ur[NV - 1];
-  double v2l = 0.0, v2r = 0.0, vnl = 0.0, vnr = 0.0;
+  const T rl = ul[0], rr = ur[0], El = ul[NV - 1], Er = ur[NV - 1];
+  T v2l = 0.0, v2r = 0.0, vnl = 0.0, vnr = 0.0;
   for (int d = 0; d < DIM; ++d) {
     vl[d] = ul[d + 1] / rl;
     vr[d] = ur[d + 1] / rr;
@@ -140,39 +151,39 @@ DG_HD void flux_hllc(double gamma, const double* ul, const double* ur, const dou
     vnl += vl[d] * nh[d];
     vnr += vr[d] * nh[d];
   }
-  const double el = El / rl, er = Er / rr;
-  const double pl = (gamma - 1.0) * (El - 0.5 * rl * v2l);
-  const double pr = (gamma - 1.0) * (Er - 0.5 * rr * v2r);
-  const double cl = sqrt(gamma * pl / rl), cr = sqrt(gamma * pr / rr);
-  const double sl = sqrt(rl), sr = sqrt(rr), ss = sl + sr;
-  double vroe2 = 0.0;
+  const T el = El / rl, er = Er / rr;
+  const T pl = (gamma - 1.0) * (El - 0.5 * rl * v2l);
+  const T pr = (gamma - 1.0) * (Er - 0.5 * rr * v2r);
+  const T cl = sqrt(gamma * pl / rl), cr = sqrt(gamma * pr / rr);
+  const T sl = sqrt(rl), sr = sqrt(rr), ss = sl + sr;
+  T vroe2 = 0.0;
   for (int d = 0; d < DIM; ++d) {
-    const double vd = (sl * vl[d] + sr * vr[d]) / ss;
+    const T vd = (sl * vl[d] + sr * vr[d]) / ss;
     vroe2 += vd * vd;
   }
-  const double vroe = (sl * vnl + sr * vnr) / ss;
-  const double Hl = (El + pl) / rl, Hr = (Er + pr) / rr;
-  const double Hroe = (sl * Hl + sr * Hr) / ss;
-  const double croe = sqrt((gamma - 1.0) * (Hroe - 0.5 * vroe2));
-  const double SL = dmin(vnl - cl, vroe - croe);
-  const double SR = dmax(vnr + cr, vroe + croe);
-  const double mL = SL - vnl, mR = SR - vnr;
+  const T vroe = (sl * vnl + sr * vnr) / ss;
+  const T Hl = (El + pl) / rl, Hr = (Er + pr) / rr;
+  const T Hroe = (sl * Hl + sr * Hr) / ss;
+  const T croe = sqrt((gamma - 1.0) * (Hroe - 0.5 * vroe2));
+  const T SL = dmin(vnl - cl, vroe - croe);
+  const T SR = dmax(vnr + cr, vroe + croe);
+  const T mL = SL - vnl, mR = SR - vnr;
   // the side whose state (or star state) gives the flux; only that side's
   // physical flux is formed
   const bool star = !(SL >= 0) && !(SR <= 0);
-  const double Ss = star ? (pr - pl + rl * vnl * mL - rr * vnr * mR) / (rl * mL - rr * mR) : 0.0;
+  const T Ss = star ? (pr - pl + rl * vnl * mL - rr * vnr * mR) / (rl * mL - rr * mR) : T(0.0);
   const bool left = star ? Ss >= 0 : SL >= 0;
-  const double* uk = left ? ul : ur;
-  double fk[NV];
+  const T* uk = left ? ul : ur;
+  T fk[NV];
   physical_flux_n<DIM>(gamma, uk, nh, fk);
   if (!star) {
     for (int i = 0; i < NV; ++i) f[i] = fk[i];
   } else {
-    const double* vk = left ? vl : vr;
-    const double rk = left ? rl : rr, vnk = left ? vnl : vnr, ek = left ? el : er;
-    const double pk = left ? pl : pr, Sk = left ? SL : SR, mk = left ? mL : mR;
-    const double rs = rk * mk / (Sk - Ss);
-    const double es = ek + (Ss - vnk) * (Ss + pk / (rk * mk));
+    const T* vk = left ? vl : vr;
+    const T rk = left ? rl : rr, vnk = left ? vnl : vnr, ek = left ? el : er;
+    const T pk = left ? pl : pr, Sk = left ? SL : SR, mk = left ? mL : mR;
+    const T rs = rk * mk / (Sk - Ss);
+    const T es = ek + (Ss - vnk) * (Ss + pk / (rk * mk));
     f[0] = fk[0] + Sk * (rs - uk[0]);
     for (int d = 0; d < DIM; ++d) f[d + 1] = fk[d + 1] + Sk * (rs * (vk[d] + (Ss - vnk) * nh[d]) - uk[d + 1]);
     f[NV - 1] = fk[NV - 1] + Sk * (rs * es - uk[NV - 1]);

@@ -351,6 +351,82 @@ DG_HD void llf_vjp_side(double gamma, const double* ul, const NodePrims<DIM>& ql
 }
 
 // ---------------------------------------------------------------------
+// HLLC: forward-mode dual numbers through the flux as physics.hpp writes it
+// (flux_hllc is templated on its scalar type), so the tangent follows the
+// branches (wave-speed bounds, star region, side) the forward evaluation
+// takes.  The adjoint builds the Jacobian's columns one input at a time.
+// ---------------------------------------------------------------------
+
+// value and derivative; arithmetic by the usual rules, comparisons on values
+struct Dual {
+  double v, d;
+  DG_HD Dual(double x = 0.0, double dx = 0.0) : v(x), d(dx) {}
+  DG_HD Dual& operator+=(Dual b) {
+    v += b.v;
+    d += b.d;
+    return *this;
+  }
+  DG_HD Dual& operator*=(Dual b) {
+    d = d * b.v + v * b.d;
+    v *= b.v;
+    return *this;
+  }
+};
+DG_HD Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+DG_HD Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+DG_HD Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
+DG_HD Dual operator*(Dual a, Dual b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+DG_HD Dual operator/(Dual a, Dual b) {
+  const double q = a.v / b.v;
+  return Dual(q, (a.d - q * b.d) / b.v);
+}
+DG_HD Dual sqrt(Dual a) {
+  const double r = sqrt(a.v);
+  return Dual(r, 0.5 * a.d / r);
+}
+DG_HD bool operator<(Dual a, Dual b) { return a.v < b.v; }
+DG_HD bool operator<=(Dual a, Dual b) { return a.v <= b.v; }
+DG_HD bool operator>=(Dual a, Dual b) { return a.v >= b.v; }
+
+// df = (dF/du_l) du_l + (dF/du_r) du_r for the HLLC flux F(u_l, u_r; n)
+template <int DIM>
+DG_HD void hllc_jvp(double gamma, const double* ul, const double* ur, const double* dul,
+                    const double* dur, const double* n, double* df) {
+  constexpr int NV = DIM + 2;
+  Dual a[NV], b[NV], f[NV];
+  for (int i = 0; i < NV; ++i) {
+    a[i] = Dual(ul[i], dul[i]);
+    b[i] = Dual(ur[i], dur[i]);
+  }
+  flux_hllc<DIM, Dual>(gamma, a, b, n, f);
+  for (int i = 0; i < NV; ++i) df[i] = f[i].d;
+}
+
+// ubar_side += (dF/du_side)^T g for the sides selected: one column of the
+// Jacobian per input component.
+template <int DIM>
+DG_HD void hllc_vjp(double gamma, const double* ul, const double* ur, const double* n,
+                    const double* g, bool with_left, bool with_right, double* ubar_l,
+                    double* ubar_r) {
+  constexpr int NV = DIM + 2;
+  for (int side = 0; side < 2; ++side) {
+    if (side == 0 ? !with_left : !with_right) continue;
+    double* ub = side == 0 ? ubar_l : ubar_r;
+    for (int j = 0; j < NV; ++j) {
+      Dual a[NV], b[NV], f[NV];
+      for (int i = 0; i < NV; ++i) {
+        a[i] = Dual(ul[i], side == 0 && i == j ? 1.0 : 0.0);
+        b[i] = Dual(ur[i], side == 1 && i == j ? 1.0 : 0.0);
+      }
+      flux_hllc<DIM, Dual>(gamma, a, b, n, f);
+      double s = 0.0;
+      for (int i = 0; i < NV; ++i) s += g[i] * f[i].d;
+      ub[j] += s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
 // Navier-Stokes: gradient variables q = (rho, v, T = p / rho) and the
 // Newtonian viscous flux (parabolic_impl.hpp grad_vars, viscous_flux_dir).
 // ---------------------------------------------------------------------

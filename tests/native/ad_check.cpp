@@ -164,6 +164,67 @@ void check_llf() {
   report(name, worst_fd, worst_dot, 1e-6, 1e-13);
 }
 
+// HLLC through dual numbers: states drawn so that every branch occurs (the
+// supersonic sides and the star region on either side of the contact).
+template <int DIM>
+void check_hllc() {
+  constexpr int NV = DIM + 2;
+  double worst_fd = 0.0, worst_dot = 0.0;
+  for (int trial = 0; trial < 400; ++trial) {
+    double ul[NV], ur[NV], dl[NV], dr[NV], n[DIM], g[NV];
+    random_state<DIM>(ul);
+    random_state<DIM>(ur);
+    // a bulk velocity along the normal, up to supersonic either way
+    for (int k = 0; k < DIM; ++k) n[k] = uni(-1, 1);
+    const double shift = uni(-4.0, 4.0);
+    for (int s = 0; s < 2; ++s) {
+      double* u = s == 0 ? ul : ur;
+      double m2 = 0.0;
+      for (int k = 0; k < DIM; ++k) m2 += u[k + 1] * u[k + 1];
+      const double p = 0.4 * (u[NV - 1] - 0.5 * m2 / u[0]);
+      m2 = 0.0;
+      for (int k = 0; k < DIM; ++k) {
+        u[k + 1] += u[0] * shift * n[k];
+        m2 += u[k + 1] * u[k + 1];
+      }
+      u[NV - 1] = p / 0.4 + 0.5 * m2 / u[0];
+    }
+    for (int i = 0; i < NV; ++i) {
+      dl[i] = uni(-1, 1);
+      dr[i] = uni(-1, 1);
+      g[i] = uni(-1, 1);
+    }
+    double df[NV];
+    hllc_jvp<DIM>(1.4, ul, ur, dl, dr, n, df);
+    const double h = 1e-7;
+    double lp[NV], lm[NV], rp[NV], rm[NV], fp[NV], fm[NV];
+    for (int i = 0; i < NV; ++i) {
+      lp[i] = ul[i] + h * dl[i];
+      lm[i] = ul[i] - h * dl[i];
+      rp[i] = ur[i] + h * dr[i];
+      rm[i] = ur[i] - h * dr[i];
+    }
+    flux_hllc<DIM>(1.4, lp, rp, n, fp);
+    flux_hllc<DIM>(1.4, lm, rm, n, fm);
+    double scale = 1e-300;
+    for (int i = 0; i < NV; ++i) scale = std::fmax(scale, std::fabs(df[i]));
+    for (int i = 0; i < NV; ++i)
+      worst_fd = std::fmax(worst_fd, rel((fp[i] - fm[i]) / (2 * h), df[i], scale));
+    double bl[NV] = {}, br[NV] = {};
+    hllc_vjp<DIM>(1.4, ul, ur, n, g, true, true, bl, br);
+    double lhs = 0.0, rhs = 0.0, mag = 0.0;
+    for (int i = 0; i < NV; ++i) {
+      lhs += g[i] * df[i];
+      rhs += bl[i] * dl[i] + br[i] * dr[i];
+      mag += std::fabs(bl[i] * dl[i]) + std::fabs(br[i] * dr[i]);
+    }
+    worst_dot = std::fmax(worst_dot, std::fabs(lhs - rhs) / mag);
+  }
+  char name[64];
+  std::snprintf(name, sizeof(name), "hllc%dD", DIM);
+  report(name, worst_fd, worst_dot, 1e-6, 1e-13);
+}
+
 template <int DIM>
 void visc(const double* u, const double g[DIM][DIM + 2], double f[DIM][DIM + 2]) {
   double q[DIM + 2];
@@ -262,6 +323,8 @@ int main() {
   check_chandrashekar<3>(true);
   check_llf<2>();
   check_llf<3>();
+  check_hllc<2>();
+  check_hllc<3>();
   check_viscous<2>();
   check_viscous<3>();
   return failures ? 1 : 0;

@@ -1,8 +1,12 @@
 """Tangent and adjoint of the shock-capturing right-hand side with the
 blending factor alpha frozen: the blend (1 - alpha) split form + alpha FV
-with the LLF subcell flux (volume_flux.hpp:449-630, :821-869), on the
-Kelvin-Helmholtz (2D) and Sedov-with-LLF-subcells (3D) configurations, each
-with troubled and untroubled elements.
+(volume_flux.hpp:449-630, :821-869) with LLF, or HLLC, as both the surface
+and the subcell flux, on the Kelvin-Helmholtz (2D) and Sedov (3D, LLF)
+configurations and, for HLLC, the Kelvin-Helmholtz state in a uniform drift
+(2D) and a Gaussian bump of density and pressure in a uniform flow (3D),
+each with troubled and untroubled elements.  The drift keeps HLLC's
+contact speed away from zero on every face, where the flux switches from
+the left star state to the right one.
 
 Checks:
   * the forward records each stage's alpha (dgsem_get_stage_alpha) and a
@@ -23,7 +27,13 @@ x-velocity and sound speed are constant, so the LLF wave speed's two max()
 arguments tie on every x-direction subcell face.  The sound speeds tie only
 up to the rounding of the recomputed pressure, a kink some 1e-16 away, which
 a central difference straddles and a derivative cannot see.  The
-perturbation moves every such kink far from the state.  The finite
+perturbation moves every such kink far from the state, and it is periodic
+on the box: a jump across the periodic boundary in the x-velocity alone
+would leave the sound speeds on both sides equal up to rounding, with a
+nonzero jump for the wave speed to multiply.  No velocity component
+vanishes at a node either, where |v.n| has its kink: the shear layer's
+x-velocity is zero on the element faces y = +-0.5, and the x-velocity
+perturbation is nonzero there.  The finite
 differences use h = 1e-7, which keeps the O(h) error of an exact tie
 (equal traces on element faces) below 2e-7.
 """
@@ -38,38 +48,60 @@ from gpu_pairs import make_pair
 pytestmark = pytest.mark.gpu
 
 CASES = [
-    # dim, N, cells, alpha_max, geometry
-    (2, 3, (8, 8), 0.002, dg.GEOMETRY_NODAL),
-    (2, 3, (8, 8), 0.5, dg.GEOMETRY_AFFINE),
-    (2, 4, (6, 5), 0.5, dg.GEOMETRY_NODAL),
-    (3, 3, (4, 4, 4), 0.5, dg.GEOMETRY_NODAL),
-    (3, 2, (4, 4, 4), 0.5, dg.GEOMETRY_AFFINE),
+    # dim, N, cells, alpha_max, geometry, surface and subcell flux
+    (2, 3, (8, 8), 0.002, dg.GEOMETRY_NODAL, dg.FLUX_LLF),
+    (2, 3, (8, 8), 0.5, dg.GEOMETRY_AFFINE, dg.FLUX_LLF),
+    (2, 4, (6, 5), 0.5, dg.GEOMETRY_NODAL, dg.FLUX_LLF),
+    (3, 3, (4, 4, 4), 0.5, dg.GEOMETRY_NODAL, dg.FLUX_LLF),
+    (3, 2, (4, 4, 4), 0.5, dg.GEOMETRY_AFFINE, dg.FLUX_LLF),
+    (2, 3, (8, 8), 0.5, dg.GEOMETRY_NODAL, dg.FLUX_HLLC),
+    (3, 3, (4, 4, 4), 0.5, dg.GEOMETRY_NODAL, dg.FLUX_HLLC),
 ]
 
 
 def ids(c):
-    return f"{c[0]}d-N{c[1]}-amax{c[3]}-{'nodal' if c[4] == dg.GEOMETRY_NODAL else 'affine'}"
+    return (f"{c[0]}d-N{c[1]}-amax{c[3]}-{'nodal' if c[4] == dg.GEOMETRY_NODAL else 'affine'}"
+            f"-{'hllc' if c[5] == dg.FLUX_HLLC else 'llf'}")
 
 
-def perturbed_khi(x, gamma=1.4):
+def perturbed_khi(x, gamma=1.4, drift=(0.0, 0.0)):
     """The Kelvin-Helmholtz state (reference layout) with smooth
-    perturbations of density, velocity and pressure; x: [e][dof][2]."""
+    perturbations of density, velocity and pressure and a uniform drift
+    velocity added; x: [e][dof][2]."""
     X, Y = x[..., 0], x[..., 1]
     B = np.tanh(15 * Y + 7.5) - np.tanh(15 * Y - 7.5)
     rho = (0.5 + 0.75 * B) * (1 + 0.05 * np.sin(2 * np.pi * X + 0.7) * np.sin(np.pi * Y + 0.3))
-    vx = 0.5 * (B - 1) + 0.03 * np.cos(np.pi * X + 0.4 * Y)
-    vy = 0.1 * np.sin(2 * np.pi * X) + 0.02 * np.sin(np.pi * Y + 0.9)
+    vx = 0.5 * (B - 1) + 0.03 * np.cos(np.pi * X + 0.4) * np.cos(np.pi * Y + 0.3) + drift[0]
+    vy = 0.1 * np.sin(2 * np.pi * X) + 0.02 * np.sin(np.pi * Y + 0.9) + drift[1]
     p = 1.0 + 0.05 * np.cos(2 * np.pi * X + 1.1) * np.sin(2 * np.pi * Y + 0.2)
     E = p / (gamma - 1) + 0.5 * rho * (vx * vx + vy * vy)
     return np.stack([rho, rho * vx, rho * vy, E], axis=-1)
 
 
+def drifting_bump(x, gamma=1.4):
+    """A Gaussian bump of density and pressure at the centre of [-1, 1]^3
+    in a uniform flow with smooth periodic perturbations; x: [e][dof][3]."""
+    X, Y, Z = x[..., 0], x[..., 1], x[..., 2]
+    g = np.exp(-(X * X + Y * Y + Z * Z) / 0.08)
+    rho = 1 + 0.5 * g + 0.05 * np.sin(np.pi * X + 0.3) * np.sin(np.pi * Y + 0.5)
+    vx = 0.5 + 0.05 * np.sin(np.pi * Y + 0.2)
+    vy = 0.4 + 0.05 * np.sin(np.pi * Z + 0.6)
+    vz = 0.3 + 0.05 * np.sin(np.pi * X + 1.0)
+    p = 1 + 3 * g
+    E = p / (gamma - 1) + 0.5 * rho * (vx * vx + vy * vy + vz * vz)
+    return np.stack([rho, rho * vx, rho * vy, rho * vz, E], axis=-1)
+
+
 def pair(case):
-    dim, N, cells, amax, geo = case
+    dim, N, cells, amax, geo, flux = case
     box = ([-1.0] * dim, [1.0] * dim)
-    oc, gs = make_pair(dim, N, cells, *box, geometry=geo, volume=dg.SHOCK_CAPTURING, amax=amax,
-                       amin=0.001)
-    u = perturbed_khi(oc.coordinates()) if dim == 2 else oc.init_sedov()
+    oc, gs = make_pair(dim, N, cells, *box, geometry=geo, volume=dg.SHOCK_CAPTURING, surface=flux,
+                       fv=flux, amax=amax, amin=0.001)
+    x = oc.coordinates()
+    if flux == dg.FLUX_HLLC:
+        u = perturbed_khi(x, drift=(0.7, 0.3)) if dim == 2 else drifting_bump(x)
+    else:
+        u = perturbed_khi(x) if dim == 2 else oc.init_sedov()
     return oc, gs, u
 
 
@@ -141,7 +173,10 @@ def test_adjoint_dot_product(gpu, case):
     assert abs(lhs - rhs) <= 1e-12 * scale
 
 
-@pytest.mark.parametrize("case", [CASES[1], CASES[3]], ids=[ids(CASES[1]), ids(CASES[3])])
+STEP_CASES = [CASES[1], CASES[3], CASES[6]]
+
+
+@pytest.mark.parametrize("case", STEP_CASES, ids=[ids(c) for c in STEP_CASES])
 def test_step_adjoint_dot_product(gpu, case):
     _, gs, u0 = pair(case)
     u = gs.from_reference_layout(u0)
@@ -210,3 +245,27 @@ def test_checkpointed_khi_gradient_matches_frozen_finite_differences(gpu, amax):
         worst = max(worst, abs(fd - ad) / abs(ad))
     print(f"case khi objective amax={amax}: J {J:.6e}, worst rel {worst:.2e}", flush=True)
     assert worst <= 1e-6
+
+
+@pytest.mark.parametrize("what", ["ns-hllc-surface", "ns-hllc-subcells", "floor"])
+def test_refused_configurations(gpu, what):
+    """Navier-Stokes with an HLLC flux and the positivity floor are not
+    differentiated: the derivative entry points return DGSEM_E_UNSUPPORTED
+    and leave the output untouched."""
+    kw = dict(dim=2, polydeg=3, cells=(4, 4), domain_left=(-1, -1), domain_right=(1, 1),
+              volume_integral=dg.SHOCK_CAPTURING)
+    if what == "floor":
+        kw.update(positivity_floor=1e-10)
+    else:
+        kw.update(equation=dg.NAVIER_STOKES, mu=1e-3, prandtl=0.72)
+        kw["surface_flux" if what == "ns-hllc-surface" else "fv_flux"] = dg.FLUX_HLLC
+    gs = dg.StructuredSolver(dg.SolverConfig(**kw))
+    u = gs.from_reference_layout(perturbed_khi(gs.node_coordinates()))
+    v = torch.ones_like(u)
+    out = torch.full_like(u, 7.0)
+    with pytest.raises(dg.DgsemError, match="not differentiated|built for the LLF"):
+        gs.rhs_jvp(u, v, out)
+    with pytest.raises(dg.DgsemError, match="not differentiated|built for the LLF"):
+        gs.rhs_vjp(u, v, out)
+    torch.cuda.synchronize()
+    assert bool((out == 7.0).all())
