@@ -30,6 +30,7 @@ EXPORTED = (
     "dgsem_batch_get_tuning", "dgsem_batch_set_tuning", "dgsem_batch_autotune",
     "dgsem_rhs_jvp", "dgsem_rhs_vjp", "dgsem_ssprk3_step_jvp", "dgsem_ssprk3_step_vjp",
     "dgsem_kinetic_energy", "dgsem_kinetic_energy_grad", "dgsem_analyze",
+    "dgsem_analyze_noalloc",
 )
 
 
@@ -108,6 +109,7 @@ def load(path: str | None = None):
     L.dgsem_kinetic_energy.argtypes = [vp, vp, vp]
     L.dgsem_kinetic_energy_grad.argtypes = [vp, vp, C.c_double, vp]
     L.dgsem_analyze.argtypes = [vp, vp, vp]
+    L.dgsem_analyze_noalloc.argtypes = [vp, vp, vp]
     for name in EXPORTED:
         if name not in ("dgsem_config_default", "dgsem_last_error", "dgsem_batch_last_error"):
             getattr(L, name).restype = C.c_int

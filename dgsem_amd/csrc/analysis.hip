@@ -6,6 +6,7 @@
 // step, in one pass over the state (include/dgsem_amd.h, dgsem_analyze).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "../../include/dgsem_amd.h"
@@ -18,11 +19,25 @@ constexpr int kBlock = 256;
 constexpr int kMaxBlocks = 1024;
 constexpr int kOut = 5;  // volume, kinetic energy, entropy, non-finite count, max speed
 
+// The 1D Gauss-Lobatto weights, passed by value: a node's tensor-product
+// weight is formed in the kernel as w_i w_j (w_k), the same products in the
+// same order as the host table, so it needs no device buffer.
+struct Weights1D {
+  double w[dgsem::kMaxN1];
+};
+
+// Partial sums of the analysis blocks for dgsem_analyze_noalloc: one set of
+// kOut values per block of the grid-stride reduction, every slot written by
+// its block before the final kernel reads it, so nothing needs clearing.
+// Each GPU holds its own copy for the life of the process; host code reaches
+// it only through the address hipGetSymbolAddress returns for that device.
+static __device__ __align__(256) double scratch[kMaxBlocks * kOut];
+
 // Each block reduces a fixed grid-stride set of nodes in a fixed tree order,
 // so the sums do not depend on scheduling.
 __global__ void __launch_bounds__(kBlock)
-    analyze_kernel(int dim, int nn, const double* __restrict__ wq, int nodal, double gamma,
-                   dgsem::KParams p, double* __restrict__ partial) {
+    analyze_kernel(int dim, int n, int nn, Weights1D w1, int nodal, double gamma, dgsem::KParams p,
+                   double* __restrict__ partial) {
   __shared__ double s[kOut][kBlock];
   const int nv = dim + 2;
   const int64_t total = p.E * nn;
@@ -43,7 +58,11 @@ __global__ void __launch_bounds__(kBlock)
       continue;
     }
     const double ij = nodal ? p.invJ[e * nn + node] : p.invJ[e];
-    const double w = wq[node] / fabs(ij);
+    // node = (k n + j) n + i
+    const int ix = node % n, jx = (node / n) % n, kx = node / (n * n);
+    double wq = w1.w[ix] * w1.w[jx];
+    if (dim == 3) wq = wq * w1.w[kx];
+    const double w = wq / fabs(ij);
     const double rho = q[0];
     double m2 = 0.0;
     for (int k = 0; k < dim; ++k) m2 += q[k + 1] * q[k + 1];
@@ -97,35 +116,87 @@ __global__ void __launch_bounds__(kBlock)
   if (threadIdx.x < kOut) out[threadIdx.x] = s[threadIdx.x][0];
 }
 
-}  // namespace
-
-extern "C" {
-
-int dgsem_analyze(dgsem_handle* h, const double* u, double* out) {
-  if (!h || !h->d_Ja || !u || !out) return DGSEM_E_INVALID;
-  int rc = dgsem::prepare_weights(h);
-  if (rc) return rc;
-  if (!h->d_an_partial &&
-      hipMalloc((void**)&h->d_an_partial, (size_t)kMaxBlocks * kOut * sizeof(double)) != hipSuccess) {
-    h->err = "hipMalloc failed";
-    return DGSEM_E_ALLOC;
-  }
+// Enqueues the two analysis kernels with the block partials in `partial`
+// (kMaxBlocks * kOut doubles).
+int launch_analyze(dgsem_handle* h, const double* u, double* partial, double* out) {
   dgsem::KParams p = h->base;
   p.u = u;
+  Weights1D w1{};
+  for (int i = 0; i < h->basis.n; ++i) w1.w[i] = h->basis.weights[i];
   const int64_t total = h->E * h->nn;
   int64_t nb = (total + kBlock - 1) / kBlock;
   nb = nb < 1 ? 1 : (nb > kMaxBlocks ? kMaxBlocks : nb);
   const bool nodal = h->cfg.geometry == DGSEM_GEOMETRY_NODAL;
   hipLaunchKernelGGL(analyze_kernel, dim3((unsigned)nb), dim3(kBlock), 0, h->stream, h->cfg.dim,
-                     h->nn, h->d_wq, nodal ? 1 : 0, h->cfg.gamma, p, h->d_an_partial);
-  hipLaunchKernelGGL(analyze_final_kernel, dim3(1), dim3(kBlock), 0, h->stream, h->d_an_partial,
-                     (int)nb, out);
+                     h->basis.n, h->nn, w1, nodal ? 1 : 0, h->cfg.gamma, p, partial);
+  hipLaunchKernelGGL(analyze_final_kernel, dim3(1), dim3(kBlock), 0, h->stream, partial, (int)nb,
+                     out);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     h->err = std::string("analyze launch: ") + hipGetErrorString(e);
     return DGSEM_E_HIP;
   }
   return DGSEM_OK;
+}
+
+// Device address of `scratch` on each device, looked up once per device
+// with that device current.
+constexpr int kMaxDevices = 64;
+double* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mutex;
+
+int scratch_address(dgsem_handle* h, double** ptr) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    h->err = std::string("hipGetDevice: ") + hipGetErrorString(e);
+    return DGSEM_E_HIP;
+  }
+  if (dev != h->device) {
+    h->err = "dgsem_analyze_noalloc: the handle's device must be the current device";
+    return DGSEM_E_INVALID;
+  }
+  if (dev < 0 || dev >= kMaxDevices) {
+    h->err = "dgsem_analyze_noalloc: device index out of range";
+    return DGSEM_E_INVALID;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* a = nullptr;
+    e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !a) {
+      h->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return DGSEM_E_HIP;
+    }
+    g_scratch_ptr[dev] = static_cast<double*>(a);
+  }
+  *ptr = g_scratch_ptr[dev];
+  return DGSEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dgsem_analyze(dgsem_handle* h, const double* u, double* out) {
+  if (!h || !h->d_Ja || !u || !out) return DGSEM_E_INVALID;
+  if (!h->d_an_partial &&
+      hipMalloc((void**)&h->d_an_partial, (size_t)kMaxBlocks * kOut * sizeof(double)) != hipSuccess) {
+    h->err = "hipMalloc failed";
+    return DGSEM_E_ALLOC;
+  }
+  return launch_analyze(h, u, h->d_an_partial, out);
+}
+
+int dgsem_analyze_noalloc(dgsem_handle* h, const double* u, double* out) {
+  if (!h || !h->d_Ja || !u || !out) return DGSEM_E_INVALID;
+  if (h->basis.n > dgsem::kMaxN1) {
+    h->err = "dgsem_analyze_noalloc: polynomial degree beyond the built maximum";
+    return DGSEM_E_INVALID;
+  }
+  double* partial = nullptr;
+  const int rc = scratch_address(h, &partial);
+  return rc != DGSEM_OK ? rc : launch_analyze(h, u, partial, out);
 }
 
 }  // extern "C"

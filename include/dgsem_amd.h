@@ -273,6 +273,18 @@ int dgsem_kinetic_energy_grad(dgsem_handle* h, const double* u, double scale, do
  * Reduced in a fixed order. */
 int dgsem_analyze(dgsem_handle* h, const double* u, double* out);
 
+/* dgsem_analyze without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  The same kernels and the same results, bit
+ * for bit; the per-block partial sums go to a static device array of
+ * 1024 x 5 doubles (40 KiB) that each GPU holds until the process ends,
+ * instead of a buffer dgsem_analyze allocates on its first call.  Every size
+ * fits it (the reduction is grid-stride with at most 1024 blocks); the
+ * handle's device must be the current device.  Calls on one GPU share the
+ * array: calls that overlap on one GPU (several streams or host threads)
+ * race, so serialise them. */
+int dgsem_analyze_noalloc(dgsem_handle* h, const double* u, double* out);
+
 /* Writes the device state u (internal layout) to a VTK XML unstructured-grid
  * file of Lagrange cells, as VTUOutputObserver::write_vtu does
  * (vtu_output_observer.hpp:62-214, vtu_output_observer_impl.cpp:15-117):
