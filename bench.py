@@ -161,11 +161,17 @@ def cpu_baseline(N: int, cells: int, steps: int, warmup: int) -> dict:
             "host": host}
 
 
-def batch_measure(ncases: int = 256, steps: int = 100, seed: int = 2024) -> dict:
+def batch_measure(ncases: int = 256, steps: int = 100, seed: int = 2024, sequential: bool = True) -> dict:
     """Ensemble of 2D Kelvin-Helmholtz cases (grids from {8,16,32,64}^2, N from
     {2,3,4,5}): `steps` SSPRK3 steps of the autotuned batch (graph replay of
     one persistent launch per stage) against the same steps run case after
-    case with the single-case handles; per-case results compared bitwise."""
+    case with the single-case handles; per-case results compared bitwise.
+    sequential=False times the batch alone (for profiling it).
+
+    Algorithmic HBM bytes of one stage launch, per node: the state (4
+    doubles) read once at stage 1 and with the base state at stages 2 and 3
+    (1 + 2/3 reads on average), the result written once, and the nodal
+    metric terms Ja (2 x 2) and 1/J."""
     import torch
 
     from dgsem_amd import ensemble
@@ -184,6 +190,16 @@ def batch_measure(ncases: int = 256, steps: int = 100, seed: int = 2024) -> dict
     ev1.record()
     torch.cuda.synchronize()
     t_batch = ev0.elapsed_time(ev1)
+    stage_bytes_ = nodes * (4 * 8 * (1.0 + 2.0 / 3.0 + 1.0) + 5 * 8)
+    stage_ms = t_batch / (3 * steps)
+    res = {"workload": f"khi2d_ensemble_{ncases}_cases_{steps}_steps_seed{seed}",
+           "nodes": nodes, "batched_ms": t_batch, "stage_ms": stage_ms,
+           "bytes_per_stage": stage_bytes_,
+           "hbm": {"achieved": stage_bytes_ / (stage_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": stage_bytes_ / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+           "dof_updates_per_s": nodes * 3 * steps / (t_batch * 1e-3), "tuning": tuning}
+    if not sequential:
+        return res
     singles = []
     for i in range(ncases):
         s = batch.case(i)
@@ -198,10 +214,8 @@ def batch_measure(ncases: int = 256, steps: int = 100, seed: int = 2024) -> dict
     torch.cuda.synchronize()
     t_seq = ev0.elapsed_time(ev1)
     identical = all(torch.equal(batch.view(ub, i), singles[i][1]) for i in range(ncases))
-    return {"workload": f"khi2d_ensemble_{ncases}_cases_{steps}_steps_seed{seed}",
-            "nodes": nodes, "batched_ms": t_batch, "sequential_ms": t_seq,
-            "speedup": t_seq / t_batch, "bit_identical": identical,
-            "dof_updates_per_s": nodes * 3 * steps / (t_batch * 1e-3), "tuning": tuning}
+    res.update({"sequential_ms": t_seq, "speedup": t_seq / t_batch, "bit_identical": identical})
+    return res
 
 
 def adjoint_measure(cells: int = 16, N: int = 3, steps: int = 200, every: int = 20) -> dict:
@@ -383,6 +397,9 @@ def main() -> int:
                          "seconds so that the timed window runs at steady clocks")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-ensemble and adjoint measurements")
+    ap.add_argument("--batch-only", type=int, default=0, metavar="STEPS",
+                    help="time only the batched ensemble for STEPS SSPRK3 steps (no sequential "
+                         "comparison) and print its figures")
     ap.add_argument("--ns-n7", action="store_true",
                     help="also time the Navier-Stokes TGV at N = 7 on 32^3 elements")
     ap.add_argument("--profile-copy-mib", type=int, default=0,
@@ -402,6 +419,18 @@ def main() -> int:
     import torch
 
     import dgsem_amd as dg
+
+    if args.batch_only > 0:
+        res = {"batch": batch_measure(steps=args.batch_only, sequential=False)}
+        if args.profile_copy_mib > 0:
+            a = torch.ones(args.profile_copy_mib * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+            b = torch.empty_like(a)
+            for _ in range(3):
+                torch.mul(a, 1.0, out=b)
+            torch.cuda.synchronize()
+            res["copy_calibration_bytes"] = a.numel() * 8
+        print(json.dumps(res))
+        return 0
 
     N, cells = args.polydeg, args.cells
     geometry = dg.GEOMETRY_NODAL if args.geometry == "nodal" else dg.GEOMETRY_AFFINE

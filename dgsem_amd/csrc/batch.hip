@@ -35,6 +35,7 @@ struct dgsem_batch {
   // derived launch state
   dgsem::BatchTile* d_tiles = nullptr;
   int ntiles = 0;
+  dgsem::BatchRanges ranges{};  // degree ranges of the sorted tile list
   size_t lds = 0;
   int grid_eff = 0;
   bool tiles_valid = false;
@@ -77,6 +78,7 @@ dgsem::BatchLaunch base_launch(const dgsem_batch* b) {
   dgsem::BatchLaunch L{};
   L.tiles = b->d_tiles;
   L.ntiles = b->ntiles;
+  L.ranges = b->ranges;
   L.grid = b->grid_eff;
   L.lds_bytes = b->lds;
   L.shock_capturing = b->sc;
@@ -107,10 +109,19 @@ int build_tiles(dgsem_batch* b) {
       tiles.push_back(tl);
     }
   }
+  // by degree, highest first (the kernel runs one loop per degree), then by
+  // cost, so that each workgroup starts with its largest tiles
   std::stable_sort(tiles.begin(), tiles.end(), [](const dgsem::BatchTile& x, const dgsem::BatchTile& y) {
+    if (x.polydeg != y.polydeg) return x.polydeg > y.polydeg;
     return tile_cost(x.polydeg, x.nvalid) > tile_cost(y.polydeg, y.nvalid);
   });
   if (tiles.size() > (size_t)INT32_MAX) return bfail(b, DGSEM_E_INVALID, "too many tiles");
+  // begin[N]: the first tile of degree N or lower
+  for (int N = 7; N >= 0; --N) {
+    size_t k = 0;
+    while (k < tiles.size() && tiles[k].polydeg > N) ++k;
+    b->ranges.begin[N] = (int)k;
+  }
   hipError_t e = hipStreamSynchronize(b->stream);
   if (e != hipSuccess) return bhip(b, e, "hipStreamSynchronize");
   hipFree(b->d_tiles);

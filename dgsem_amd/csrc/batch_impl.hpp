@@ -3,15 +3,22 @@
 //
 // The host cuts every case into tiles of at most EPW(N) consecutive
 // elements (EPW(N) = BLOCK / LINES(N) element slots fit one workgroup) and
-// sorts the tiles by estimated cost.  Each workgroup walks the tile list
-// with a grid-size stride; a tile names its case, whose KParams (state,
-// geometry, basis and stage coefficients) sit in a device array, and its
-// polynomial degree, which selects the instantiation of the element body.
-// The element body is the same source as the single-case kernel
-// (element_body.inc), so every case gets bit-identical results to its own
-// single-case run.  Degrees 1..5 are built: one kernel holds every degree's
-// body, so its register budget is that of the largest, and the N = 7 body
-// alone would cut the whole kernel to one wave per SIMD.
+// sorts the tiles by degree, highest first, and by estimated cost within a
+// degree.  Each workgroup walks the tile list with a grid-size stride; a
+// tile names its case, whose KParams (state, geometry, basis and stage
+// coefficients) sit in a device array, and the degree's loop it falls in
+// selects the instantiation of the element body.  The element body is the
+// same source as the single-case kernel (element_body.inc), so every case
+// gets bit-identical results to its own single-case run.  Degrees 1..5 are
+// built: one kernel holds every degree's body, so its register budget is
+// that of the largest, and the N = 7 body alone would cut the whole kernel
+// to one wave per SIMD.
+//
+// batch_inst.hip is compiled with machine-level loop-invariant code motion
+// off (-mllvm -disable-machine-licm): with it, the FP64 literals of every
+// body (series and polynomial coefficients) are hoisted out of the tile
+// loops and held in registers across the whole kernel, which made the
+// kernel spill.  Without it they are materialised where they are used.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -54,24 +61,33 @@ __device__ __forceinline__ void batch_tile(const KParams& p, const int64_t e0, c
 #include "element_body.inc"
 }
 
-template <int BLOCK, bool SC, bool NODAL, int SF, int FF>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
-    batch_kernel(const BatchTile* __restrict__ tiles, int ntiles,
-                 const KParams* __restrict__ params) {
-  extern __shared__ double smem[];
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+// The tiles of one degree: those of [begin, end) whose index is congruent
+// to blockIdx.x modulo the grid, so that the loops of all degrees together
+// visit the degree-sorted list with one grid-size stride.
+template <int N, int BLOCK, bool SC, bool NODAL, int SF, int FF>
+__device__ __forceinline__ void batch_range(const BatchTile* __restrict__ tiles, int begin, int end,
+                                            const KParams* __restrict__ params, double* smem) {
+  const int g = (int)gridDim.x;
+  for (int t = begin + (((int)blockIdx.x - begin) % g + g) % g; t < end; t += g) {
     const BatchTile tile = tiles[t];
-    const KParams& p = params[tile.case_id];
-    switch (tile.polydeg) {
-      case 1: batch_tile<1, BLOCK, SC, NODAL, SF, FF>(p, tile.e0, tile.nvalid, smem); break;
-      case 2: batch_tile<2, BLOCK, SC, NODAL, SF, FF>(p, tile.e0, tile.nvalid, smem); break;
-      case 3: batch_tile<3, BLOCK, SC, NODAL, SF, FF>(p, tile.e0, tile.nvalid, smem); break;
-      case 4: batch_tile<4, BLOCK, SC, NODAL, SF, FF>(p, tile.e0, tile.nvalid, smem); break;
-      case 5: batch_tile<5, BLOCK, SC, NODAL, SF, FF>(p, tile.e0, tile.nvalid, smem); break;
-      default: break;
-    }
+    batch_tile<N, BLOCK, SC, NODAL, SF, FF>(params[tile.case_id], tile.e0, tile.nvalid, smem);
     __syncthreads();  // the next tile reuses the LDS
   }
+}
+
+// One loop per degree, each with its own body, rather than one loop that
+// switches on the tile's degree: the register allocator then sees one
+// degree's live values at a time.
+template <int BLOCK, bool SC, bool NODAL, int SF, int FF>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
+    batch_kernel(const BatchTile* __restrict__ tiles, BatchRanges r,
+                 const KParams* __restrict__ params) {
+  extern __shared__ double smem[];
+  batch_range<5, BLOCK, SC, NODAL, SF, FF>(tiles, r.begin[5], r.begin[4], params, smem);
+  batch_range<4, BLOCK, SC, NODAL, SF, FF>(tiles, r.begin[4], r.begin[3], params, smem);
+  batch_range<3, BLOCK, SC, NODAL, SF, FF>(tiles, r.begin[3], r.begin[2], params, smem);
+  batch_range<2, BLOCK, SC, NODAL, SF, FF>(tiles, r.begin[2], r.begin[1], params, smem);
+  batch_range<1, BLOCK, SC, NODAL, SF, FF>(tiles, r.begin[1], r.begin[0], params, smem);
 }
 
 template <int BLOCK, bool SC, bool NODAL, int SF, int FF>
@@ -93,7 +109,7 @@ hipError_t launch_batch_t(const BatchLaunch& b, hipStream_t stream) {
   }
   if (b.ntiles <= 0) return hipSuccess;
   hipLaunchKernelGGL(kern, dim3((unsigned)b.grid), dim3(BLOCK), b.lds_bytes, stream, b.tiles,
-                     b.ntiles, b.params);
+                     b.ranges, b.params);
   return hipGetLastError();
 }
 

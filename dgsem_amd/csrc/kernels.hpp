@@ -171,9 +171,16 @@ struct BatchTile {
   int64_t e0;      // first element of the tile within its case
 };
 
+// The tile list is sorted by degree, highest first: the tiles of degree N
+// are [begin[N], begin[N - 1]), and begin[0] is the number of tiles.
+struct BatchRanges {
+  int begin[8];
+};
+
 struct BatchLaunch {
   const BatchTile* tiles;
   int ntiles;
+  BatchRanges ranges;
   const KParams* params;  // one per case
   int grid;               // persistent workgroups
   size_t lds_bytes;       // dynamic LDS: the largest tile footprint in the batch
