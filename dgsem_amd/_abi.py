@@ -11,6 +11,7 @@ EULER, NAVIER_STOKES = 0, 1
 SPLIT_FORM, SHOCK_CAPTURING = 0, 1
 FLUX_CHANDRASHEKAR = 0
 FLUX_LLF, FLUX_HLLC = 0, 1
+FLUX_CENTRAL = 2  # volume, surface or subcell flux (2D)
 GEOMETRY_NODAL, GEOMETRY_AFFINE = 0, 1
 BC_PERIODIC, BC_SLIP_WALL, BC_OUTFLOW, BC_DIRICHLET = 0, 1, 2, 3
 BC_NS_NOSLIP_WALL, BC_NS_SLIP_WALL, BC_MIXED_DIRICHLET_SLIP_WALL = 4, 5, 6

@@ -162,6 +162,7 @@ void fill_params(dgsem_handle* h) {
   p.E = h->E;
   p.surface_flux = c.surface_flux;
   p.fv_flux = c.fv_flux;
+  p.volume_flux = c.volume_flux;
   p.gamma = c.gamma;
   p.inv_gm1 = 1.0 / (c.gamma - 1.0);
   p.alpha_max = c.alpha_max;
@@ -250,10 +251,18 @@ int check_config(dgsem_handle* h, const dgsem_config& c) {
     return fail(h, DGSEM_E_INVALID, "Navier-Stokes needs mu >= 0 and Pr > 0");
   if (c.volume_integral != DGSEM_VOLUME_SPLIT_FORM && c.volume_integral != DGSEM_VOLUME_SHOCK_CAPTURING)
     return fail(h, DGSEM_E_INVALID, "unknown volume integral");
-  if (c.volume_flux != DGSEM_FLUX_CHANDRASHEKAR) return fail(h, DGSEM_E_INVALID, "unknown volume flux");
-  if ((c.surface_flux != DGSEM_FLUX_LLF && c.surface_flux != DGSEM_FLUX_HLLC) ||
-      (c.fv_flux != DGSEM_FLUX_LLF && c.fv_flux != DGSEM_FLUX_HLLC))
-    return fail(h, DGSEM_E_INVALID, "unknown surface or FV flux");
+  if (c.volume_flux != DGSEM_FLUX_CHANDRASHEKAR && c.volume_flux != DGSEM_FLUX_CENTRAL)
+    return fail(h, DGSEM_E_INVALID, "unknown volume flux");
+  for (int kind : {c.surface_flux, c.fv_flux})
+    if (kind != DGSEM_FLUX_LLF && kind != DGSEM_FLUX_HLLC && kind != DGSEM_FLUX_CENTRAL)
+      return fail(h, DGSEM_E_INVALID, "unknown surface or FV flux");
+  // CentralFlux takes a normal in 2D only (numerical_flux.hpp:247-257); the
+  // kernels that read flux kinds at run time are the line kernels (N <= 5)
+  if (dgsem::uses_central_flux(c)) {
+    if (c.dim != 2)
+      return fail(h, DGSEM_E_UNSUPPORTED, "CentralFlux is built for 2D meshes (as in the reference)");
+    if (c.polydeg > 5) return fail(h, DGSEM_E_UNSUPPORTED, "CentralFlux is built for N <= 5");
+  }
   if (c.geometry != DGSEM_GEOMETRY_NODAL && c.geometry != DGSEM_GEOMETRY_AFFINE)
     return fail(h, DGSEM_E_INVALID, "unknown geometry mode");
   if (!(c.gamma > 1.0)) return fail(h, DGSEM_E_INVALID, "gamma must exceed 1");

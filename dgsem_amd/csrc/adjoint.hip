@@ -204,6 +204,8 @@ int ad_prepare(dgsem_handle* h) {
       (c.surface_flux != DGSEM_FLUX_LLF || (sc && c.fv_flux != DGSEM_FLUX_LLF)))
     return afail(h, DGSEM_E_UNSUPPORTED,
                  "the Navier-Stokes tangent and adjoint are built for the LLF fluxes");
+  if (dgsem::uses_central_flux(c))
+    return afail(h, DGSEM_E_UNSUPPORTED, "CentralFlux is not differentiated");
   if (c.positivity_floor > 0.0)
     return afail(h, DGSEM_E_UNSUPPORTED, "the positivity floor is not differentiated");
   for (int f = 0; f < 2 * c.dim; ++f)

@@ -183,9 +183,12 @@ int launch_stage(dgsem_batch* b, int stage, hipStream_t s) {
   return e == hipSuccess ? DGSEM_OK : bhip(b, e, "batch kernel launch");
 }
 
+// The batch kernel is built for the Chandrashekar volume flux with LLF or
+// HLLC surface and subcell fluxes.
 int check_case(const dgsem_config& c0, const dgsem_config& c) {
   return c.dim == 2 && c.equation == DGSEM_EULER && c.volume_integral == c0.volume_integral &&
-         c.surface_flux == c0.surface_flux && c.fv_flux == c0.fv_flux && c.geometry == c0.geometry;
+         c.surface_flux == c0.surface_flux && c.fv_flux == c0.fv_flux && c.geometry == c0.geometry &&
+         c.volume_flux == DGSEM_FLUX_CHANDRASHEKAR && !dgsem::uses_central_flux(c);
 }
 
 }  // namespace
@@ -203,7 +206,8 @@ int dgsem_batch_create(const dgsem_config* cfgs, int ncases, void* stream, dgsem
     if (!check_case(cfgs[0], cfgs[i]))
       return bfail(b, DGSEM_E_UNSUPPORTED,
                    "batch cases must be 2D Euler with one volume integral, flux pair and geometry "
-                   "mode (case " + std::to_string(i) + ")");
+                   "mode, the Chandrashekar volume flux and LLF or HLLC fluxes (case " +
+                       std::to_string(i) + ")");
     if (dgsem::batch_slots(256, cfgs[i].polydeg) <= 0)
       return bfail(b, DGSEM_E_UNSUPPORTED, "polynomial degree not built for batches");
   }

@@ -66,4 +66,11 @@ int launch_params(dgsem_handle* h, const KParams& p);
 // the reduction scratch on first use.
 int prepare_weights(dgsem_handle* h);
 
+// CentralFlux anywhere in the configuration: as the volume flux, the
+// surface flux or, with shock capturing, the subcell flux.
+inline bool uses_central_flux(const dgsem_config& c) {
+  return c.volume_flux == DGSEM_FLUX_CENTRAL || c.surface_flux == DGSEM_FLUX_CENTRAL ||
+         (c.volume_integral == DGSEM_VOLUME_SHOCK_CAPTURING && c.fv_flux == DGSEM_FLUX_CENTRAL);
+}
+
 }  // namespace dgsem

@@ -66,7 +66,8 @@ struct KParams {
   double* visc_out;      // written by the viscous kernel
   int64_t E;
   int mode;
-  int surface_flux, fv_flux;
+  int surface_flux, fv_flux;  // FluxKind
+  int volume_flux;            // VolumeFluxKind (read by the run-time flux kernels only)
   double c1, c2, c3;
   double gamma, inv_gm1;
   double alpha_max, alpha_min, atol, threshold, sfac;

@@ -27,7 +27,12 @@ using std::fabs;
 using std::log;
 using std::sqrt;
 
-enum FluxKind : int { kFluxLLF = 0, kFluxHLLC = 1 };
+// Numerical flux kinds (include/dgsem_amd.h DGSEM_FLUX_*).  kFluxAny is a
+// template value only: the kernel instantiation that reads the volume,
+// surface and subcell flux kinds from its parameters at run time.
+enum FluxKind : int { kFluxLLF = 0, kFluxHLLC = 1, kFluxCentral = 2, kFluxAny = 3 };
+// volume flux kinds
+enum VolumeFluxKind : int { kVolumeChandrashekar = 0, kVolumeCentral = 2 };
 
 DG_HD double dmax(double a, double b) { return (a < b) ? b : a; }
 DG_HD double dmin(double a, double b) { return (b < a) ? b : a; }
@@ -191,11 +196,24 @@ DG_HD void flux_hllc(double gamma, const T* ul, const T* ur, const double* n, T*
   for (int i = 0; i < NV; ++i) f[i] *= nrm;
 }
 
+// CentralFlux with a normal, numerical_flux.hpp:247-257 (2D in the
+// reference): 0.5 (F(u_L).n + F(u_R).n), numerical_flux_detail.hpp:23-32.
+template <int DIM>
+DG_HD void flux_central(double gamma, const double* ul, const double* ur, const double* n, double* f) {
+  constexpr int NV = DIM + 2;
+  double fl[NV], fr[NV];
+  physical_flux_n<DIM>(gamma, ul, n, fl);
+  physical_flux_n<DIM>(gamma, ur, n, fr);
+  for (int i = 0; i < NV; ++i) f[i] = 0.5 * (fl[i] + fr[i]);
+}
+
 template <int DIM>
 DG_HD void surface_flux(int kind, double gamma, const double* ul, const double* ur, const double* n,
                         double* f) {
   if (kind == kFluxHLLC)
     flux_hllc<DIM>(gamma, ul, ur, n, f);
+  else if (kind == kFluxCentral)
+    flux_central<DIM>(gamma, ul, ur, n, f);
   else
     flux_llf<DIM>(gamma, ul, ur, n, f);
 }

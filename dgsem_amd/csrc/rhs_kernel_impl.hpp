@@ -287,7 +287,8 @@ __device__ __forceinline__ void boundary_state(const KParams& p, int face, int64
 
 // SF / FF: surface and subcell-FV flux kinds (kFluxLLF, kFluxHLLC) as
 // template parameters, so a kernel only carries the register footprint of
-// the fluxes it uses.
+// the fluxes it uses; SF = FF = kFluxAny is the 2D instantiation that reads
+// the volume, surface and subcell flux kinds from p (CentralFlux).
 // BCS: some face of the (2D) mesh has a non-periodic boundary condition.
 template <int DIM, int N, int EPW, bool SC, bool NODAL, int SF, int FF, bool BCS, bool BOX>
 __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
@@ -325,7 +326,9 @@ namespace detail {
 
 template <int DIM, int N, bool SC, bool NODAL, int SF, int FF, bool BCS>
 hipError_t launch_one(const KParams& p, hipStream_t stream) {
-  if constexpr (use_node_kernel<DIM, N, SC, BCS>()) return launch_node_kernel<DIM, N, NODAL, SF>(p, stream);
+  static_assert(SF != kFluxAny || (DIM == 2 && N <= 5), "run-time fluxes: 2D line kernels");
+  if constexpr (SF != kFluxAny && use_node_kernel<DIM, N, SC, BCS>())
+    return launch_node_kernel<DIM, N, NODAL, SF>(p, stream);
   constexpr int EPW = elems_per_block<DIM, N>();
   constexpr int BLOCK = EPW * Shape<DIM, N>::LINES;
   const int64_t nb = (p.E + EPW - 1) / EPW;
@@ -346,6 +349,17 @@ hipError_t launch_one(const KParams& p, hipStream_t stream) {
 
 template <int DIM, int N, bool SC, bool NODAL, bool BCS>
 hipError_t launch_flux(const KParams& p, hipStream_t s) {
+  // CentralFlux anywhere (2D, N <= 5; the ABI refuses other cases): the
+  // instantiation that selects every flux at run time
+  if constexpr (DIM == 2 && N <= 5) {
+    if (p.volume_flux == kVolumeCentral || p.surface_flux == kFluxCentral ||
+        (SC && p.fv_flux == kFluxCentral))
+      return launch_one<DIM, N, SC, NODAL, kFluxAny, kFluxAny, BCS>(p, s);
+  } else {
+    if (p.volume_flux == kVolumeCentral || p.surface_flux == kFluxCentral ||
+        (SC && p.fv_flux == kFluxCentral))
+      return hipErrorInvalidValue;
+  }
   const bool shllc = p.surface_flux == kFluxHLLC;
   if constexpr (SC) {
     const bool fhllc = p.fv_flux == kFluxHLLC;

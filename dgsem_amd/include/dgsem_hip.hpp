@@ -265,9 +265,13 @@ normal_t<Eq> axis(std::size_t dim) {
 
 // LaxFriedrichsFlux (numerical_flux.hpp:39-228): 0.5 (F_L + F_R) -
 // 0.5 lambda (u_R - u_L) with the equation's wave speed.
+// Each flux trait carries `kind`, its DGSEM_FLUX_* value as a surface or
+// subcell flux (-1: not one), and `volume_kind`, its value as the volume
+// flux of a split form (-1: not one the kernels take).
 template <class Eq>
 struct LaxFriedrichsFlux {
   static constexpr int kind = DGSEM_FLUX_LLF;
+  static constexpr int volume_kind = -1;
   using value_type = typename Eq::value_type;
   static constexpr std::size_t NDIMS = Eq::NDIMS, NVARS = Eq::NVARS;
   static flux_detail::state_t<Eq> numerical_flux(const Eq& eq, const flux_detail::state_t<Eq>& uL,
@@ -294,6 +298,7 @@ struct LaxFriedrichsFlux {
 template <class Eq>
 struct HLLCFlux {
   static constexpr int kind = DGSEM_FLUX_HLLC;
+  static constexpr int volume_kind = -1;
   static constexpr std::size_t NDIMS = Eq::NDIMS, NVARS = Eq::NVARS;
   static flux_detail::state_t<Eq> numerical_flux(const Eq& eq, const flux_detail::state_t<Eq>& uL,
                                                  const flux_detail::state_t<Eq>& uR,
@@ -314,7 +319,8 @@ struct HLLCFlux {
 // kinetic-energy-preserving two-point flux of the split-form volume integral.
 template <class Eq>
 struct ChandrashekarFlux {
-  static constexpr int kind = DGSEM_FLUX_CHANDRASHEKAR;
+  static constexpr int kind = -1;
+  static constexpr int volume_kind = DGSEM_FLUX_CHANDRASHEKAR;
   static constexpr std::size_t NDIMS = Eq::NDIMS, NVARS = Eq::NVARS;
   static flux_detail::state_t<Eq> numerical_flux(const Eq& eq, const flux_detail::state_t<Eq>& uL,
                                                  const flux_detail::state_t<Eq>& uR,
@@ -333,12 +339,13 @@ struct ChandrashekarFlux {
   }
 };
 
-// CentralFlux (numerical_flux.hpp:230-258): 0.5 (F_L + F_R).  A host-side
-// trait only: the kernels are built for the LLF and HLLC surface fluxes and
-// the Chandrashekar volume flux, so a solver rejects it at compile time.
+// CentralFlux (numerical_flux.hpp:230-258): 0.5 (F_L + F_R).  The reference
+// defines its normal form for 2D only; the kernels take it as the volume,
+// surface or subcell flux of 2D solvers up to N = 5.
 template <class Eq>
 struct CentralFlux {
-  static constexpr int kind = -1;
+  static constexpr int kind = DGSEM_FLUX_CENTRAL;
+  static constexpr int volume_kind = DGSEM_FLUX_CENTRAL;
   static constexpr std::size_t NDIMS = Eq::NDIMS, NVARS = Eq::NVARS;
   static flux_detail::state_t<Eq> numerical_flux(const Eq& eq, const flux_detail::state_t<Eq>& uL,
                                                  const flux_detail::state_t<Eq>& uR,
@@ -367,7 +374,7 @@ struct HGIndicator {};
 template <class Basis, class Eq, template <class> class NumericFlux>
 struct VolumeIntegralSplitForm {
   static constexpr int policy = DGSEM_VOLUME_SPLIT_FORM;
-  static constexpr int volume_flux = NumericFlux<Eq>::kind;
+  static constexpr int volume_flux = NumericFlux<Eq>::volume_kind;
   static constexpr int fv_flux = DGSEM_FLUX_LLF;
 };
 
@@ -377,7 +384,7 @@ template <class Basis, class Eq, template <class> class NumericFlux, template <c
           class Indicator>
 struct VolumeIntegralShockCapturingHG : VolumeIntegralShockCapturingBase {
   static constexpr int policy = DGSEM_VOLUME_SHOCK_CAPTURING;
-  static constexpr int volume_flux = NumericFlux<Eq>::kind;
+  static constexpr int volume_flux = NumericFlux<Eq>::volume_kind;
   static constexpr int fv_flux = FvFlux<Eq>::kind;
 };
 
@@ -539,8 +546,12 @@ class StructuredSolver {
   static_assert(BoundarySetType::NFACES == 2 * NDIMS,
                 "StructuredSolver requires one boundary condition per mesh face");
   static_assert(SurfaceFlux::kind >= 0 && VolumeFlux::volume_flux >= 0 && VolumeFlux::fv_flux >= 0,
-                "the GPU kernels take the LLF or HLLC surface and subcell fluxes and the "
-                "Chandrashekar volume flux");
+                "the GPU kernels take the LLF, HLLC or central surface and subcell fluxes and the "
+                "Chandrashekar or central volume flux");
+  static_assert(NDIMS == 2 || (SurfaceFlux::kind != DGSEM_FLUX_CENTRAL &&
+                               VolumeFlux::volume_flux != DGSEM_FLUX_CENTRAL &&
+                               VolumeFlux::fv_flux != DGSEM_FLUX_CENTRAL),
+                "CentralFlux takes a normal in 2D only (numerical_flux.hpp:247-257)");
   using solution = Solution<Mesh, BasisT, Equations>;
   using ElementCache = StructuredElementContainer<value_type, NDIMS>;
 

@@ -35,6 +35,7 @@ class SolverConfig:
     mu: float | None = None
     prandtl: float | None = None
     volume_integral: int = _abi.SPLIT_FORM
+    volume_flux: int = _abi.FLUX_CHANDRASHEKAR
     surface_flux: int = _abi.FLUX_LLF
     fv_flux: int = _abi.FLUX_LLF
     alpha_max: float = 0.5
@@ -62,6 +63,7 @@ class SolverConfig:
         if self.prandtl is not None:
             c.prandtl = self.prandtl
         c.volume_integral = self.volume_integral
+        c.volume_flux = self.volume_flux
         c.surface_flux = self.surface_flux
         c.fv_flux = self.fv_flux
         c.alpha_max = self.alpha_max

@@ -46,6 +46,10 @@ extern "C" {
 #define DGSEM_FLUX_CHANDRASHEKAR 0 /* volume flux: ChandrashekarFlux */
 #define DGSEM_FLUX_LLF 0           /* surface / FV flux: LaxFriedrichsFlux */
 #define DGSEM_FLUX_HLLC 1          /* surface / FV flux: HLLC with a normal */
+/* volume, surface or FV flux: CentralFlux, 0.5 (F(u_L) + F(u_R)).n
+ * (numerical_flux.hpp:230-258, which defines the normal form for 2D only);
+ * 2D meshes, N <= 5; not in batches or the derivatives */
+#define DGSEM_FLUX_CENTRAL 2
 
 /* geometry representation on the device */
 #define DGSEM_GEOMETRY_NODAL 0  /* per-node metric terms exactly as the
@@ -86,9 +90,9 @@ typedef struct dgsem_config {
   double mu;               /* dynamic viscosity (Navier-Stokes) */
   double prandtl;          /* Prandtl number (Navier-Stokes) */
   int volume_integral;     /* DGSEM_VOLUME_* */
-  int volume_flux;         /* DGSEM_FLUX_CHANDRASHEKAR */
-  int surface_flux;        /* DGSEM_FLUX_LLF or DGSEM_FLUX_HLLC */
-  int fv_flux;             /* subcell FV flux for shock capturing */
+  int volume_flux;         /* DGSEM_FLUX_CHANDRASHEKAR or DGSEM_FLUX_CENTRAL */
+  int surface_flux;        /* DGSEM_FLUX_LLF, DGSEM_FLUX_HLLC or DGSEM_FLUX_CENTRAL */
+  int fv_flux;             /* subcell FV flux for shock capturing (the same kinds) */
   double alpha_max;        /* set_indicator_parameters: default 0.5 */
   double alpha_min;        /* default 0.001 */
   int alpha_smooth;        /* stored, unused (as in the reference) */

@@ -27,7 +27,7 @@
 struct orc_case {
   int dim, N, n, nn, nvars, nx, ny, nz;
   int64_t E;
-  int eq, volume, fv_flux, surface_flux, nthreads;
+  int eq, volume, fv_flux, surface_flux, volume_flux, nthreads;
   double gamma, mu, prandtl, kappa;
   double alpha_max, alpha_min, floor_value;
   double nodes[MAXN], weights[MAXN], inv_weights[MAXN];
@@ -505,10 +505,24 @@ void orc_flux_hllc(int dim, double g, const double* ul, const double* ur,
   for (int i = 0; i < nv; ++i) f[i] *= nrm;
 }
 
+/* CentralFlux::numerical_flux with a normal (numerical_flux.hpp:247-257,
+ * 2D in the reference): eq.flux(u, normal) of both states, then
+ * central_flux_from_physical_fluxes, 0.5 (f_L + f_R)
+ * (numerical_flux_detail.hpp:23-32). */
+void orc_flux_central(int dim, double g, const double* ul, const double* ur,
+                      const double* n, double* f) {
+  double fl[5], fr[5];
+  orc_physical_flux(dim, g, ul, n, fl);
+  orc_physical_flux(dim, g, ur, n, fr);
+  for (int i = 0; i < dim + 2; ++i) f[i] = 0.5 * (fl[i] + fr[i]);
+}
+
 static void surface_flux_fn(const orc_case* c, int kind, const double* ul,
                             const double* ur, const double* n, double* f) {
   if (kind == ORC_FLUX_HLLC)
     orc_flux_hllc(c->dim, c->gamma, ul, ur, n, f);
+  else if (kind == ORC_FLUX_CENTRAL)
+    orc_flux_central(c->dim, c->gamma, ul, ur, n, f);
   else
     orc_flux_llf(c->dim, c->gamma, ul, ur, n, f);
 }
@@ -833,6 +847,8 @@ void orc_set_volume(orc_case* c, int volume, int fv_flux, double amax,
   c->alpha_min = amin;
 }
 void orc_set_surface_flux(orc_case* c, int sf) { c->surface_flux = sf; }
+
+void orc_set_volume_flux(orc_case* c, int vf) { c->volume_flux = vf; }
 void orc_set_positivity_floor(orc_case* c, double v) { c->floor_value = v; }
 void orc_set_threads(orc_case* c, int t) { c->nthreads = t; }
 int64_t orc_num_elements(const orc_case* c) { return c->E; }
@@ -1103,7 +1119,10 @@ static void split_form(const orc_case* c, int64_t e, const double* u,
             for (int v = 0; v < nv; ++v) ul[v] = U(u, e, dof_l, v);
             for (int comp = 0; comp < d; ++comp)
               nrm[comp] = 0.5 * (JA(e, dof_i, dir, comp) + JA(e, dof_l, dir, comp));
-            orc_flux_chandrashekar(d, c->gamma, ui, ul, nrm, f);
+            if (c->volume_flux == ORC_VOLUME_CENTRAL)
+              orc_flux_central(d, c->gamma, ui, ul, nrm, f);
+            else
+              orc_flux_chandrashekar(d, c->gamma, ui, ul, nrm, f);
             const int a = idx[dir];
             for (int v = 0; v < nv; ++v) {
               U(du, e, dof_i, v) += alpha * c->Dsplit[a * n + l] * f[v];

@@ -24,7 +24,9 @@ extern "C" {
 
 enum { ORC_EULER = 0, ORC_NAVIER_STOKES = 1 };
 enum { ORC_SPLIT_FORM = 0, ORC_SHOCK_CAPTURING = 1 };
-enum { ORC_FLUX_LLF = 0, ORC_FLUX_HLLC = 1 };
+enum { ORC_FLUX_LLF = 0, ORC_FLUX_HLLC = 1, ORC_FLUX_CENTRAL = 2 };
+/* volume fluxes of the split form */
+enum { ORC_VOLUME_CHANDRASHEKAR = 0, ORC_VOLUME_CENTRAL = 2 };
 
 typedef struct orc_case orc_case;
 
@@ -48,6 +50,8 @@ void orc_set_equation(orc_case* c, int eq, double gamma, double mu,
 void orc_set_volume(orc_case* c, int volume, int fv_flux, double alpha_max,
                     double alpha_min);
 void orc_set_surface_flux(orc_case* c, int surface_flux);
+/* the split form's two-point flux: ORC_VOLUME_* (default Chandrashekar) */
+void orc_set_volume_flux(orc_case* c, int volume_flux);
 void orc_set_positivity_floor(orc_case* c, double floor_value);
 void orc_set_threads(orc_case* c, int nthreads);
 
@@ -127,6 +131,8 @@ void orc_flux_llf(int dim, double gamma, const double* ul, const double* ur,
                   const double* normal, double* f);
 void orc_flux_hllc(int dim, double gamma, const double* ul, const double* ur,
                    const double* normal, double* f);
+void orc_flux_central(int dim, double gamma, const double* ul, const double* ur,
+                      const double* normal, double* f);
 void orc_physical_flux(int dim, double gamma, const double* u,
                        const double* normal, double* f);
 

@@ -21,7 +21,8 @@ _lib = None
 
 EULER, NAVIER_STOKES = 0, 1
 SPLIT_FORM, SHOCK_CAPTURING = 0, 1
-FLUX_LLF, FLUX_HLLC = 0, 1
+FLUX_LLF, FLUX_HLLC, FLUX_CENTRAL = 0, 1, 2
+VOLUME_CHANDRASHEKAR, VOLUME_CENTRAL = 0, 2
 BC_PERIODIC, BC_SLIP_WALL, BC_OUTFLOW, BC_DIRICHLET = 0, 1, 2, 3
 BC_NS_NOSLIP_WALL, BC_NS_SLIP_WALL, BC_MIXED_DIRICHLET_SLIP_WALL = 4, 5, 6
 
@@ -50,6 +51,7 @@ def lib():
     L.orc_set_equation.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double]
     L.orc_set_volume.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double]
     L.orc_set_surface_flux.argtypes = [vp, C.c_int]
+    L.orc_set_volume_flux.argtypes = [vp, C.c_int]
     L.orc_set_positivity_floor.argtypes = [vp, C.c_double]
     L.orc_set_threads.argtypes = [vp, C.c_int]
     L.orc_num_elements.restype = C.c_int64
@@ -79,7 +81,7 @@ def lib():
     L.orc_viscous_flux.argtypes = [vp, _dp, _dp, C.c_int, _dp]
     L.orc_ln_mean.restype = C.c_double
     L.orc_ln_mean.argtypes = [C.c_double, C.c_double]
-    for name in ("orc_flux_chandrashekar", "orc_flux_llf", "orc_flux_hllc"):
+    for name in ("orc_flux_chandrashekar", "orc_flux_llf", "orc_flux_hllc", "orc_flux_central"):
         getattr(L, name).argtypes = [C.c_int, C.c_double, _dp, _dp, _dp, _dp]
     L.orc_physical_flux.argtypes = [C.c_int, C.c_double, _dp, _dp, _dp]
     L.orc_kinetic_energy.restype = C.c_double
@@ -126,6 +128,10 @@ def flux_hllc(dim, gamma, ul, ur, n):
     return _flux("orc_flux_hllc", dim, gamma, ul, ur, n)
 
 
+def flux_central(dim, gamma, ul, ur, n):
+    return _flux("orc_flux_central", dim, gamma, ul, ur, n)
+
+
 def physical_flux(dim, gamma, u, n):
     f = np.zeros(dim + 2)
     lib().orc_physical_flux(dim, gamma, np.ascontiguousarray(u, dtype=np.float64),
@@ -139,7 +145,7 @@ class Case:
     def __init__(self, dim, N, cells, left, right, periodic=None, *, eq=EULER,
                  gamma=1.4, mu=None, prandtl=None, volume=SPLIT_FORM, fv_flux=FLUX_LLF,
                  alpha_max=0.5, alpha_min=0.001, surface_flux=FLUX_LLF,
-                 positivity_floor=0.0, threads=0):
+                 positivity_floor=0.0, threads=0, volume_flux=VOLUME_CHANDRASHEKAR):
         self.dim, self.N = dim, N
         cells = list(cells) + [1] * (3 - len(cells))
         self.cells = tuple(cells[:dim])
@@ -158,6 +164,7 @@ class Case:
         L.orc_set_equation(self._h, eq, gamma, mu, prandtl)
         L.orc_set_volume(self._h, volume, fv_flux, alpha_max, alpha_min)
         L.orc_set_surface_flux(self._h, surface_flux)
+        L.orc_set_volume_flux(self._h, volume_flux)
         L.orc_set_positivity_floor(self._h, positivity_floor)
         L.orc_set_threads(self._h, threads)
         self.E = int(L.orc_num_elements(self._h))

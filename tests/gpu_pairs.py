@@ -11,15 +11,17 @@ def rel_l2(a, b):
 
 
 def make_pair(dim, N, cells, left, right, *, geometry=dg.GEOMETRY_NODAL, volume=dg.SPLIT_FORM,
-              surface=dg.FLUX_LLF, fv=dg.FLUX_LLF, amax=0.5, amin=0.001, floor=0.0, mu=None):
+              surface=dg.FLUX_LLF, fv=dg.FLUX_LLF, amax=0.5, amin=0.001, floor=0.0, mu=None,
+              vflux=dg.FLUX_CHANDRASHEKAR):
     """Oracle case and GPU solver of one configuration; mu: Navier-Stokes
     with that viscosity and Prandtl number 0.72 (else compressible Euler)."""
     ns = {} if mu is None else dict(eq=O.NAVIER_STOKES, mu=mu, prandtl=0.72)
     oc = O.Case(dim, N, cells, left, right, volume=volume, surface_flux=surface, fv_flux=fv,
-                alpha_max=amax, alpha_min=amin, positivity_floor=floor, **ns)
+                alpha_max=amax, alpha_min=amin, positivity_floor=floor, volume_flux=vflux, **ns)
     ns = {} if mu is None else dict(equation=dg.NAVIER_STOKES, mu=mu, prandtl=0.72)
     cfg = dg.SolverConfig(dim=dim, polydeg=N, cells=cells, domain_left=left, domain_right=right,
-                          volume_integral=volume, surface_flux=surface, fv_flux=fv, alpha_max=amax,
+                          volume_integral=volume, volume_flux=vflux, surface_flux=surface, fv_flux=fv,
+                          alpha_max=amax,
                           alpha_min=amin, positivity_floor=floor, geometry=geometry, **ns)
     gs = dg.StructuredSolver(cfg)
     if geometry == dg.GEOMETRY_AFFINE:
