@@ -104,60 +104,94 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 // Hennemann-Gassner modal indicator for the EPW elements of the block,
 // indicator.hpp:337-417 (2D) / :454-574 (3D), including the 3D total-energy
 // sum that skips the modes (i = N, j < N, k < N) and gamma fixed at 1.4.
+//
+// The modal coefficients come from one sweep of V^-1 per direction.  A
+// sweep gives each thread one line of n nodes (EPW n^(DIM-1) lines), so every
+// index into V^-1 is a compile-time constant and its entries are scalar
+// loads; the first sweep forms the indicator variable of its line's nodes
+// itself.  Each output is the sum over the line in increasing order, with
+// the same fused multiply-adds as a node-parallel sweep.  The scratch arrays
+// A and B use the padded node positions of the state planes (lds_pos), so
+// the line reads and writes hit distinct banks.  The energy sums follow the
+// reference's order, one thread per element.
 template <int DIM, int N, int EPW, int BLOCK>
 __device__ void block_alpha(const KParams& p, const double* s_u, double* s_w, double* s_alpha,
                             int nvalid) {
   using S = Shape<DIM, N>;
-  constexpr int n = S::n, NN = S::NN, NV = S::NV;
+  constexpr int n = S::n, NN = S::NN, NV = S::NV, PN = S::PN;
+  constexpr int LN = NN / n;  // lines per direction and element
+  static_assert(2 * EPW * PN <= EPW * S::SP, "indicator scratch in the primitives' LDS");
   double* A = s_w;
-  double* B = s_w + EPW * NN;
+  double* B = s_w + EPW * PN;
+  const double* V = p.Vinv;
   const int tid = threadIdx.x;
-  for (int i = tid; i < EPW * NN; i += BLOCK) {
-    const int slot = i / NN, node = i % NN;
-    double s[NV];
-    for (int v = 0; v < NV; ++v) s[v] = s_u[slot * S::SU + v * S::PN + lds_pos<N>(node)];
-    A[i] = slot < nvalid ? indicator_variable<DIM>(s) : 0.0;
+  // sweep along xi, lines of fixed (j, k), from the indicator variable
+  for (int t = tid; t < EPW * LN; t += BLOCK) {
+    const int slot = t / LN, row = (t % LN) * n;  // the line's node i = 0
+    double x[n];
+#pragma unroll
+    for (int ii = 0; ii < n; ++ii) {
+      double s[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) s[v] = s_u[slot * S::SU + v * PN + lds_pos<N>(row + ii)];
+      x[ii] = slot < nvalid ? indicator_variable<DIM>(s) : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < n; ++i) {
+      double r = 0.0;
+#pragma unroll
+      for (int ii = 0; ii < n; ++ii) r += V[i * n + ii] * x[ii];
+      B[slot * PN + lds_pos<N>(row + i)] = r;
+    }
   }
   __syncthreads();
-  // sweep along xi
-  for (int i = tid; i < EPW * NN; i += BLOCK) {
-    const int node = i % NN, ix = node % n, row = i - ix;
-    double r = 0.0;
-    for (int ii = 0; ii < n; ++ii) r += p.Vinv[ix * n + ii] * A[row + ii];
-    B[i] = r;
-  }
-  __syncthreads();
-  // sweep along eta
-  for (int i = tid; i < EPW * NN; i += BLOCK) {
-    const int node = i % NN, jx = (node / n) % n, base = i - jx * n;
-    double r = 0.0;
-    for (int jj = 0; jj < n; ++jj) r += p.Vinv[jx * n + jj] * B[base + jj * n];
-    A[i] = r;
+  // sweep along eta, lines of fixed (i, k)
+  for (int t = tid; t < EPW * LN; t += BLOCK) {
+    const int slot = t / LN, l = t % LN;
+    const int base = (l / n) * n * n + l % n;
+    double x[n];
+#pragma unroll
+    for (int jj = 0; jj < n; ++jj) x[jj] = B[slot * PN + lds_pos<N>(base + jj * n)];
+#pragma unroll
+    for (int j = 0; j < n; ++j) {
+      double r = 0.0;
+#pragma unroll
+      for (int jj = 0; jj < n; ++jj) r += V[j * n + jj] * x[jj];
+      A[slot * PN + lds_pos<N>(base + j * n)] = r;
+    }
   }
   __syncthreads();
   const double* modal = A;
   if constexpr (DIM == 3) {
-    for (int i = tid; i < EPW * NN; i += BLOCK) {
-      const int node = i % NN, kx = node / (n * n), base = i - kx * n * n;
-      double r = 0.0;
-      for (int kk = 0; kk < n; ++kk) r += p.Vinv[kx * n + kk] * A[base + kk * n * n];
-      B[i] = r;
+    // sweep along zeta, lines of fixed (i, j)
+    for (int t = tid; t < EPW * LN; t += BLOCK) {
+      const int slot = t / LN, base = t % LN;
+      double x[n];
+#pragma unroll
+      for (int kk = 0; kk < n; ++kk) x[kk] = A[slot * PN + lds_pos<N>(base + kk * n * n)];
+#pragma unroll
+      for (int k = 0; k < n; ++k) {
+        double r = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < n; ++kk) r += V[k * n + kk] * x[kk];
+        B[slot * PN + lds_pos<N>(base + k * n * n)] = r;
+      }
     }
     __syncthreads();
     modal = B;
   }
   if (tid < nvalid) {
-    const double* m = modal + tid * NN;
+    const double* m = modal + tid * PN;
+    auto M = [&](int k, int j, int i) { return m[lds_pos<N>((k * n + j) * n + i)]; };
     double total = 0.0, clip1 = 0.0, clip2 = 0.0;
     if constexpr (DIM == 2) {
       for (int j = 0; j < n; ++j)
-        for (int i = 0; i < n; ++i) total += m[j * n + i] * m[j * n + i];
+        for (int i = 0; i < n; ++i) total += M(0, j, i) * M(0, j, i);
       for (int j = 0; j + 1 < n; ++j)
-        for (int i = 0; i + 1 < n; ++i) clip1 += m[j * n + i] * m[j * n + i];
+        for (int i = 0; i + 1 < n; ++i) clip1 += M(0, j, i) * M(0, j, i);
       for (int j = 0; j + 2 < n; ++j)
-        for (int i = 0; i + 2 < n; ++i) clip2 += m[j * n + i] * m[j * n + i];
+        for (int i = 0; i + 2 < n; ++i) clip2 += M(0, j, i) * M(0, j, i);
     } else {
-      auto M = [&](int k, int j, int i) { return m[(k * n + j) * n + i]; };
       for (int k = 0; k < n - 2; ++k)
         for (int j = 0; j < n - 2; ++j)
           for (int i = 0; i < n - 2; ++i) clip2 += M(k, j, i) * M(k, j, i);
