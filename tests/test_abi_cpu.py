@@ -82,3 +82,35 @@ def test_direction_kind_applies_to_both_faces_without_gpu():
     h = ctypes.c_void_p()
     assert lib.dgsem_create(ctypes.byref(c), None, ctypes.byref(h)) == -2
     lib.dgsem_destroy(h)
+
+
+@pytest.mark.parametrize("dim,N,changes,rc,word", [
+    (2, 3, dict(polydeg=0), -1, b"polydeg"),
+    (2, 3, dict(polydeg=8), -1, b"polydeg"),
+    (3, 6, {}, -2, b"N=6"),
+    (3, 1, {}, -2, b"3D N=1"),
+    (2, 3, dict(gamma=1.0), -1, b"gamma"),
+    (2, 3, dict(equation=7), -1, b"equation"),
+    (2, 3, dict(volume_integral=5), -1, b"volume integral"),
+    (2, 3, dict(volume_flux=1), -1, b"volume flux"),
+    (2, 3, dict(surface_flux=9), -1, b"surface or FV flux"),
+    (2, 3, dict(fv_flux=-1), -1, b"surface or FV flux"),
+    (2, 3, dict(geometry=4), -1, b"geometry"),
+    (3, 3, dict(surface_flux=2), -2, b"CentralFlux"),
+    (3, 3, dict(volume_flux=2), -2, b"CentralFlux"),
+    (3, 3, dict(volume_integral=1, fv_flux=2), -2, b"CentralFlux"),
+    (2, 7, dict(surface_flux=2), -2, b"CentralFlux"),
+    (2, 3, dict(equation=1, mu=-1.0), -1, b"Navier-Stokes"),
+])
+def test_configuration_errors_without_gpu(dim, N, changes, rc, word):
+    """Every configuration check of dgsem_create runs before device work, so
+    it answers here with the error code and a message that names the field."""
+    lib = _abi.load()
+    c = _abi.Config()
+    lib.dgsem_config_default(ctypes.byref(c), dim, N)
+    for k, v in changes.items():
+        setattr(c, k, v)
+    h = ctypes.c_void_p()
+    assert lib.dgsem_create(ctypes.byref(c), None, ctypes.byref(h)) == rc
+    assert word in lib.dgsem_last_error(h), lib.dgsem_last_error(h)
+    lib.dgsem_destroy(h)
