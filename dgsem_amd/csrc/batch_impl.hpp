@@ -50,6 +50,10 @@ __device__ __forceinline__ void batch_tile(const KParams& p, const int64_t e0, c
   double* const s_u = smem;
   double* const s_p = smem + EPW * S::SU;
   double* const s_alpha = s_p + EPW * S::SP;
+  // the indicator works in s_p, before the primitives: the LDS is sized for
+  // the largest degree of the ensemble, and 2D kernels are LDS-bound
+  constexpr bool OVERLAP = false;
+  double* const s_ind = nullptr;
   double* const s_du = s_p;
   // The thread's roles (line, direction, node positions) do not change from
   // tile to tile; computed from a plain threadIdx.x the compiler would hoist

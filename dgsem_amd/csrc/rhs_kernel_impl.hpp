@@ -110,22 +110,64 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 // index into V^-1 is a compile-time constant and its entries are scalar
 // loads; the first sweep forms the indicator variable of its line's nodes
 // itself.  Each output is the sum over the line in increasing order, with
-// the same fused multiply-adds as a node-parallel sweep.  The scratch arrays
-// A and B use the padded node positions of the state planes (lds_pos), so
-// the line reads and writes hit distinct banks.  The energy sums follow the
-// reference's order, one thread per element.
-template <int DIM, int N, int EPW, int BLOCK>
+// the same fused multiply-adds as a node-parallel sweep.  A thread reads its
+// line before it writes it, so every sweep works in place in one scratch
+// plane per element, at the padded node positions of the state planes
+// (lds_pos): the line reads and writes hit distinct banks.  The energy sums
+// follow the reference's order, one thread per element.
+//
+// OVERLAP: the sweeps and the energy sums keep only EPW n^(DIM-1) and EPW
+// threads busy; the others meanwhile form the per-node primitives, prim(i)
+// for i < EPW NN, a slice per phase.  s_w (indicator_scratch() doubles) then
+// lies apart from the primitives' array; without OVERLAP it may be that
+// array, and prim goes unused.
+template <int DIM, int N, int EPW>
+constexpr int indicator_scratch() {
+  return EPW * Shape<DIM, N>::PN;
+}
+
+// Whether a shock-capturing kernel overlaps the primitives with the
+// indicator: only where its separate scratch leaves the LDS-bound number of
+// blocks per CU (160 KiB of LDS) unchanged -- in 2D at N = 3 it would cut
+// three blocks to two, and the step measured 23% slower -- and not with the
+// HLLC subcell flux, whose 3D N = 3 kernel measured 3% slower with it (the
+// same spill count, the code laid out differently) where the LLF kernel
+// gained 3%.
+#ifndef DGSEM_OVERLAP_HLLC
+#define DGSEM_OVERLAP_HLLC 0
+#endif
+template <int DIM, int N, int EPW, bool SC, int FF>
+constexpr bool overlap_prims() {
+  using S = Shape<DIM, N>;
+  constexpr int lds_cu = 160 * 1024;
+  constexpr int base = 8 * (EPW * (S::SU + S::SP) + EPW);
+  constexpr int with = base + 8 * indicator_scratch<DIM, N, EPW>();
+  return SC && (FF != kFluxHLLC || DGSEM_OVERLAP_HLLC) && lds_cu / with == lds_cu / base;
+}
+
+template <int DIM, int N, int EPW, int BLOCK, bool OVERLAP, class Prim>
 __device__ void block_alpha(const KParams& p, const double* s_u, double* s_w, double* s_alpha,
-                            int nvalid) {
+                            int nvalid, Prim&& prim) {
   using S = Shape<DIM, N>;
   constexpr int n = S::n, NN = S::NN, NV = S::NV, PN = S::PN;
-  constexpr int LN = NN / n;  // lines per direction and element
-  static_assert(2 * EPW * PN <= EPW * S::SP, "indicator scratch in the primitives' LDS");
+  constexpr int LN = NN / n;    // lines per direction and element
+  constexpr int T0 = EPW * LN;  // threads of a sweep
+  static_assert(!OVERLAP || T0 < BLOCK, "threads left for the primitives");
+  static_assert(OVERLAP || indicator_scratch<DIM, N, EPW>() <= EPW * S::SP,
+                "indicator scratch in the primitives' LDS");
+  constexpr int PHASES = DIM + 1;  // the sweeps and the energy sums
   double* A = s_w;
-  double* B = s_w + EPW * PN;
   const double* V = p.Vinv;
   const int tid = threadIdx.x;
+  // the primitives' slice of phase f, on the threads a sweep leaves idle
+  auto prims_slice = [&](int f) {
+    if (OVERLAP && tid >= T0) {
+      const int lo = f * (EPW * NN) / PHASES, hi = (f + 1) * (EPW * NN) / PHASES;
+      for (int i = lo + (tid - T0); i < hi; i += BLOCK - T0) prim(i);
+    }
+  };
   // sweep along xi, lines of fixed (j, k), from the indicator variable
+  prims_slice(0);
   for (int t = tid; t < EPW * LN; t += BLOCK) {
     const int slot = t / LN, row = (t % LN) * n;  // the line's node i = 0
     double x[n];
@@ -141,17 +183,18 @@ __device__ void block_alpha(const KParams& p, const double* s_u, double* s_w, do
       double r = 0.0;
 #pragma unroll
       for (int ii = 0; ii < n; ++ii) r += V[i * n + ii] * x[ii];
-      B[slot * PN + lds_pos<N>(row + i)] = r;
+      A[slot * PN + lds_pos<N>(row + i)] = r;
     }
   }
   __syncthreads();
   // sweep along eta, lines of fixed (i, k)
+  prims_slice(1);
   for (int t = tid; t < EPW * LN; t += BLOCK) {
     const int slot = t / LN, l = t % LN;
     const int base = (l / n) * n * n + l % n;
     double x[n];
 #pragma unroll
-    for (int jj = 0; jj < n; ++jj) x[jj] = B[slot * PN + lds_pos<N>(base + jj * n)];
+    for (int jj = 0; jj < n; ++jj) x[jj] = A[slot * PN + lds_pos<N>(base + jj * n)];
 #pragma unroll
     for (int j = 0; j < n; ++j) {
       double r = 0.0;
@@ -164,6 +207,7 @@ __device__ void block_alpha(const KParams& p, const double* s_u, double* s_w, do
   const double* modal = A;
   if constexpr (DIM == 3) {
     // sweep along zeta, lines of fixed (i, j)
+    prims_slice(2);
     for (int t = tid; t < EPW * LN; t += BLOCK) {
       const int slot = t / LN, base = t % LN;
       double x[n];
@@ -174,12 +218,12 @@ __device__ void block_alpha(const KParams& p, const double* s_u, double* s_w, do
         double r = 0.0;
 #pragma unroll
         for (int kk = 0; kk < n; ++kk) r += V[k * n + kk] * x[kk];
-        B[slot * PN + lds_pos<N>(base + k * n * n)] = r;
+        A[slot * PN + lds_pos<N>(base + k * n * n)] = r;
       }
     }
     __syncthreads();
-    modal = B;
   }
+  prims_slice(DIM);
   if (tid < nvalid) {
     const double* m = modal + tid * PN;
     auto M = [&](int k, int j, int i) { return m[lds_pos<N>((k * n + j) * n + i)]; };
@@ -340,6 +384,8 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
   __shared__ double s_u[EPW * S::SU];
   __shared__ double s_p[EPW * S::SP];
   __shared__ double s_alpha[EPW];
+  constexpr bool OVERLAP = overlap_prims<DIM, N, EPW, SC, FF>();
+  __shared__ double s_ind[OVERLAP ? indicator_scratch<DIM, N, EPW>() : 1];
   double* const s_du = s_p;
 
   const int tid = threadIdx.x;
