@@ -129,20 +129,14 @@ constexpr int indicator_scratch() {
 // Whether a shock-capturing kernel overlaps the primitives with the
 // indicator: only where its separate scratch leaves the LDS-bound number of
 // blocks per CU (160 KiB of LDS) unchanged -- in 2D at N = 3 it would cut
-// three blocks to two, and the step measured 23% slower -- and not with the
-// HLLC subcell flux, whose 3D N = 3 kernel measured 3% slower with it (the
-// same spill count, the code laid out differently) where the LLF kernel
-// gained 3%.
-#ifndef DGSEM_OVERLAP_HLLC
-#define DGSEM_OVERLAP_HLLC 0
-#endif
-template <int DIM, int N, int EPW, bool SC, int FF>
+// three blocks to two, and the step measured 23% slower.
+template <int DIM, int N, int EPW, bool SC>
 constexpr bool overlap_prims() {
   using S = Shape<DIM, N>;
   constexpr int lds_cu = 160 * 1024;
   constexpr int base = 8 * (EPW * (S::SU + S::SP) + EPW);
   constexpr int with = base + 8 * indicator_scratch<DIM, N, EPW>();
-  return SC && (FF != kFluxHLLC || DGSEM_OVERLAP_HLLC) && lds_cu / with == lds_cu / base;
+  return SC && lds_cu / with == lds_cu / base;
 }
 
 template <int DIM, int N, int EPW, int BLOCK, bool OVERLAP, class Prim>
@@ -384,7 +378,7 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES))
   __shared__ double s_u[EPW * S::SU];
   __shared__ double s_p[EPW * S::SP];
   __shared__ double s_alpha[EPW];
-  constexpr bool OVERLAP = overlap_prims<DIM, N, EPW, SC, FF>();
+  constexpr bool OVERLAP = overlap_prims<DIM, N, EPW, SC>();
   __shared__ double s_ind[OVERLAP ? indicator_scratch<DIM, N, EPW>() : 1];
   double* const s_du = s_p;
 
