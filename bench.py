@@ -400,6 +400,9 @@ def main() -> int:
     ap.add_argument("--batch-only", type=int, default=0, metavar="STEPS",
                     help="time only the batched ensemble for STEPS SSPRK3 steps (no sequential "
                          "comparison) and print its figures")
+    ap.add_argument("--sedov-only", type=int, default=0, metavar="STEPS",
+                    help="time only the 3D Sedov blast (32^3, N = 3, shock capturing with HLLC "
+                         "subcells) for STEPS SSPRK3 steps and print its figures")
     ap.add_argument("--ns-n7", action="store_true",
                     help="also time the Navier-Stokes TGV at N = 7 on 32^3 elements")
     ap.add_argument("--profile-copy-mib", type=int, default=0,
@@ -420,8 +423,11 @@ def main() -> int:
 
     import dgsem_amd as dg
 
-    if args.batch_only > 0:
-        res = {"batch": batch_measure(steps=args.batch_only, sequential=False)}
+    if args.batch_only > 0 or args.sedov_only > 0:
+        if args.batch_only > 0:
+            res = {"batch": batch_measure(steps=args.batch_only, sequential=False)}
+        else:
+            res = {"sedov": sedov_measure(steps=args.sedov_only)}
         if args.profile_copy_mib > 0:
             a = torch.ones(args.profile_copy_mib * (1 << 20) // 8, dtype=torch.float64, device="cuda")
             b = torch.empty_like(a)
