@@ -200,10 +200,6 @@ int alloc_once(dgsem_handle* h, double** p, size_t count) {
 int ad_prepare(dgsem_handle* h) {
   const auto& c = h->cfg;
   const bool sc = c.volume_integral == DGSEM_VOLUME_SHOCK_CAPTURING;
-  if (c.equation == DGSEM_NAVIER_STOKES &&
-      (c.surface_flux != DGSEM_FLUX_LLF || (sc && c.fv_flux != DGSEM_FLUX_LLF)))
-    return afail(h, DGSEM_E_UNSUPPORTED,
-                 "the Navier-Stokes tangent and adjoint are built for the LLF fluxes");
   if (dgsem::uses_central_flux(c))
     return afail(h, DGSEM_E_UNSUPPORTED, "CentralFlux is not differentiated");
   if (c.positivity_floor > 0.0)

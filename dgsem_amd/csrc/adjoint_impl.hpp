@@ -783,8 +783,9 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(c
   }
 }
 
-// Navier-Stokes handles take the LLF fluxes only (the ABI checks it), so
-// their kernels are built for LLF alone.
+// The Navier-Stokes derivatives run the viscous kernels first (they form
+// a.dvisc, which the RHS kernels add); every kernel takes the flux kinds as
+// template arguments.
 template <int DIM, int N, bool NODAL, bool SC, bool ADJ>
 hipError_t launch_ad_t(bool ns, const AdParams& a, hipStream_t s) {
   constexpr int EPW = ad_elems_per_block<DIM, N>();
@@ -796,35 +797,34 @@ hipError_t launch_ad_t(bool ns, const AdParams& a, hipStream_t s) {
     if constexpr (ADJ) {
       hipLaunchKernelGGL((vjp_visc1_kernel<DIM, N, EPW, NODAL>), grid, block, 0, s, a);
       hipLaunchKernelGGL((vjp_visc2_kernel<DIM, N, EPW, NODAL>), grid, block, 0, s, a);
-      hipLaunchKernelGGL((vjp_rhs_kernel<DIM, N, EPW, NODAL, true, SC, kFluxLLF, kFluxLLF>), grid,
-                         block, 0, s, a);
     } else {
       hipLaunchKernelGGL((jvp_grad_kernel<DIM, N, EPW, NODAL>), grid, block, 0, s, a);
       hipLaunchKernelGGL((jvp_visc_kernel<DIM, N, EPW, NODAL>), grid, block, 0, s, a);
-      hipLaunchKernelGGL((jvp_rhs_kernel<DIM, N, EPW, NODAL, true, SC, kFluxLLF, kFluxLLF>), grid,
-                         block, 0, s, a);
     }
-    return hipGetLastError();
   }
   const bool shllc = a.k.surface_flux == kFluxHLLC, fhllc = SC && a.k.fv_flux == kFluxHLLC;
-  auto go = [&](auto sf, auto ff) {
+  auto go = [&](auto nsv, auto sf, auto ff) {
+    constexpr bool NSv = decltype(nsv)::value;
     constexpr int SFv = decltype(sf)::value, FFv = decltype(ff)::value;
     if constexpr (ADJ)
-      hipLaunchKernelGGL((vjp_rhs_kernel<DIM, N, EPW, NODAL, false, SC, SFv, FFv>), grid, block, 0, s,
+      hipLaunchKernelGGL((vjp_rhs_kernel<DIM, N, EPW, NODAL, NSv, SC, SFv, FFv>), grid, block, 0, s,
                          a);
     else
-      hipLaunchKernelGGL((jvp_rhs_kernel<DIM, N, EPW, NODAL, false, SC, SFv, FFv>), grid, block, 0, s,
+      hipLaunchKernelGGL((jvp_rhs_kernel<DIM, N, EPW, NODAL, NSv, SC, SFv, FFv>), grid, block, 0, s,
                          a);
     return hipGetLastError();
   };
-  using LLF = std::integral_constant<int, kFluxLLF>;
-  using HLLC = std::integral_constant<int, kFluxHLLC>;
-  if constexpr (SC) {
-    if (shllc) return fhllc ? go(HLLC{}, HLLC{}) : go(HLLC{}, LLF{});
-    return fhllc ? go(LLF{}, HLLC{}) : go(LLF{}, LLF{});
-  } else {
-    return shllc ? go(HLLC{}, LLF{}) : go(LLF{}, LLF{});
-  }
+  auto flux = [&](auto nsv) {
+    using LLF = std::integral_constant<int, kFluxLLF>;
+    using HLLC = std::integral_constant<int, kFluxHLLC>;
+    if constexpr (SC) {
+      if (shllc) return fhllc ? go(nsv, HLLC{}, HLLC{}) : go(nsv, HLLC{}, LLF{});
+      return fhllc ? go(nsv, LLF{}, HLLC{}) : go(nsv, LLF{}, LLF{});
+    } else {
+      return shllc ? go(nsv, HLLC{}, LLF{}) : go(nsv, LLF{}, LLF{});
+    }
+  };
+  return ns ? flux(std::true_type{}) : flux(std::false_type{});
 }
 
 template <int DIM, int N, bool NODAL, bool SC>
