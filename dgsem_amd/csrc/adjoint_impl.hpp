@@ -85,6 +85,94 @@ __device__ __forceinline__ bool frozen_blend(const KParams& p, int64_t e, double
   return false;
 }
 
+// Non-periodic faces (2D meshes; the neighbour table holds -1 there).  The
+// outer state of face `face` (2 dir + upper) at boundary node bnode is the
+// one the forward's boundary_state forms (rhs_kernel_impl.hpp), with the
+// line's states read from memory.  As a map of the state it is one of
+// three kinds: a slip wall (and the inviscid part of the Navier-Stokes
+// walls) mirrors the momentum about the face, linear and symmetric in the
+// momentum; outflow takes the mean of the line's n nodes; a Dirichlet state
+// (constant or per node) does not depend on the state.
+enum class BcMap { kConstant, kMirror, kMean };
+
+template <int DIM>
+__device__ __forceinline__ BcMap bc_map(const KParams& p, int face, int64_t bnode) {
+  constexpr int NV = DIM + 2;
+  const int kind = p.bc[face];
+  if (kind == kBcMixedDirichletSlipWall && p.bc_values[face][bnode * (NV + 1) + NV] != 0.0)
+    return BcMap::kConstant;
+  if (kind == kBcSlipWall || kind == kBcNsNoSlipWall || kind == kBcNsSlipWall ||
+      kind == kBcMixedDirichletSlipWall)
+    return BcMap::kMirror;
+  return kind == kBcOutflow ? BcMap::kMean : BcMap::kConstant;
+}
+
+// Boundary node of line `line` of element e on a face of direction dir
+// (2D, elements numbered e = ie * ny + je), as the forward numbers it.
+template <int DIM, int N>
+__device__ __forceinline__ int64_t boundary_node(const KParams& p, int64_t e, int dir, int line) {
+  const int64_t along = dir == 0 ? e % p.cells[1] : e / p.cells[1];
+  return along * (N + 1) + line;
+}
+
+// The outer state; ue: the element's states in memory ([var][node]).
+template <int DIM, int N, int NN>
+__device__ void bc_outer_state(const KParams& p, int face, int64_t bnode, BcMap m, const double* ue,
+                               const int* dofs, const double* uin, const double* nrm, double* uout) {
+  constexpr int n = N + 1, NV = DIM + 2;
+  if (m == BcMap::kMirror) {
+    const double rho = uin[0];
+    double v[DIM], vn = 0.0, nn = 0.0;
+    for (int c = 0; c < DIM; ++c) v[c] = uin[1 + c] / rho;
+    for (int c = 0; c < DIM; ++c) nn += nrm[c] * nrm[c];
+    for (int c = 0; c < DIM; ++c) vn += v[c] * nrm[c];
+    vn = vn / nn;
+    uout[0] = rho;
+    for (int c = 0; c < DIM; ++c) uout[1 + c] = rho * (v[c] - 2.0 * vn * nrm[c]);
+    uout[NV - 1] = uin[NV - 1];
+  } else if (m == BcMap::kMean) {
+    for (int v = 0; v < NV; ++v) {
+      double acc = 0.0;
+      for (int k = 0; k < n; ++k) acc += ue[v * NN + dofs[k]];
+      uout[v] = acc / (double)n;
+    }
+  } else {
+    const double* vals = p.bc_values[face];
+    if (p.bc[face] == kBcMixedDirichletSlipWall) {
+      for (int v = 0; v < NV; ++v) uout[v] = vals[bnode * (NV + 1) + v];
+    } else {
+      for (int v = 0; v < NV; ++v) uout[v] = vals ? vals[bnode * NV + v] : p.bc_state[face][v];
+    }
+  }
+}
+
+// Its tangent: dve the tangent's states of the element, duin the tangent at
+// the boundary node.
+template <int DIM, int N, int NN>
+__device__ void bc_outer_tangent(BcMap m, const double* dve, const int* dofs, const double* duin,
+                                 const double* nrm, double* duout) {
+  constexpr int n = N + 1, NV = DIM + 2;
+  if (m == BcMap::kMirror) {
+    double mn = 0.0, nn = 0.0;
+    for (int c = 0; c < DIM; ++c) {
+      mn += duin[1 + c] * nrm[c];
+      nn += nrm[c] * nrm[c];
+    }
+    mn = mn / nn;
+    duout[0] = duin[0];
+    for (int c = 0; c < DIM; ++c) duout[1 + c] = duin[1 + c] - 2.0 * mn * nrm[c];
+    duout[NV - 1] = duin[NV - 1];
+  } else if (m == BcMap::kMean) {
+    for (int v = 0; v < NV; ++v) {
+      double acc = 0.0;
+      for (int k = 0; k < n; ++k) acc += dve[v * NN + dofs[k]];
+      duout[v] = acc / (double)n;
+    }
+  } else {
+    for (int v = 0; v < NV; ++v) duout[v] = 0.0;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Tangent
 // ---------------------------------------------------------------------------
@@ -367,24 +455,52 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) jvp_rhs_kernel(c
         }
       }
     }
-    // LLF faces, both states and their tangents from memory
+    // faces, both states and their tangents from memory
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll 1
     for (int side = 0; side < 2; ++side) {
       const bool lower = side == 0;
       const int64_t nb = p.nbr[e * 2 * DIM + 2 * dir + side];
       double ul[NV], ur[NV], dul[NV], dur[NV], nrm[DIM], df[NV];
-      // the face's lower element contributes its last node, the upper one its first
-      const int64_t el = lower ? nb : e, er = lower ? e : nb;
-      const int dl = dofs[n - 1], dr = dofs[0];
-      load_state<DIM, NN>(p.u, el, dl, ul);
-      load_state<DIM, NN>(p.u, er, dr, ur);
-      load_state<DIM, NN>(a.v, el, dl, dul);
-      load_state<DIM, NN>(a.v, er, dr, dur);
-      // the normal of the face's upper element at its first node
-      ja(er, dr, nrm);
-      const double sg = sign_j<NODAL, DIM, NN>(p, er, dr);
-      for (int c = 0; c < DIM; ++c) nrm[c] *= sg;
+      bool boundary = false;  // non-periodic faces exist on 2D meshes only
+      if constexpr (DIM == 2) {
+        if (nb < 0) {
+          // a non-periodic face: the own trace, the normal of the own
+          // element at it, and the outer state on the far side
+          boundary = true;
+          const int od = lower ? dofs[0] : dofs[n - 1];
+          double uo[NV], duo[NV], ub[NV], dub[NV];
+          load_state<DIM, NN>(p.u, e, od, uo);
+          load_state<DIM, NN>(a.v, e, od, duo);
+          ja(e, od, nrm);
+          const double sg = sign_j<NODAL, DIM, NN>(p, e, od);
+          for (int c = 0; c < DIM; ++c) nrm[c] *= sg;
+          const int face = 2 * dir + side;
+          const int64_t bnode = boundary_node<DIM, N>(p, e, dir, line);
+          const BcMap m = bc_map<DIM>(p, face, bnode);
+          bc_outer_state<DIM, N, NN>(p, face, bnode, m, p.u + e * NV * NN, dofs, uo, nrm, ub);
+          bc_outer_tangent<DIM, N, NN>(m, a.v + e * NV * NN, dofs, duo, nrm, dub);
+          for (int v = 0; v < NV; ++v) {
+            ul[v] = lower ? ub[v] : uo[v];
+            dul[v] = lower ? dub[v] : duo[v];
+            ur[v] = lower ? uo[v] : ub[v];
+            dur[v] = lower ? duo[v] : dub[v];
+          }
+        }
+      }
+      if (!boundary) {
+        // the face's lower element contributes its last node, the upper one its first
+        const int64_t el = lower ? nb : e, er = lower ? e : nb;
+        const int dl = dofs[n - 1], dr = dofs[0];
+        load_state<DIM, NN>(p.u, el, dl, ul);
+        load_state<DIM, NN>(p.u, er, dr, ur);
+        load_state<DIM, NN>(a.v, el, dl, dul);
+        load_state<DIM, NN>(a.v, er, dr, dur);
+        // the normal of the face's upper element at its first node
+        ja(er, dr, nrm);
+        const double sg = sign_j<NODAL, DIM, NN>(p, er, dr);
+        for (int c = 0; c < DIM; ++c) nrm[c] *= sg;
+      }
       if constexpr (SF == kFluxHLLC)
         hllc_jvp<DIM>(p.gamma, ul, ur, dul, dur, nrm, df);
       else
@@ -626,6 +742,11 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(c
   }
   double ub_lo[NV], ub_hi[NV];
   for (int v = 0; v < NV; ++v) ub_lo[v] = ub_hi[v] = 0.0;
+  // an outflow face's outer state is the mean of the line: its cotangent
+  // goes to every node of the line
+  double ub_line[NV];
+  bool line_mean = false;
+  for (int v = 0; v < NV; ++v) ub_line[v] = 0.0;
   double coef = 1.0, alpha = 0.0;
   const bool troubled = valid && frozen_blend<SC>(p, e0 + slot, coef, alpha);
   if (valid) {
@@ -664,12 +785,60 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(c
         chandrashekar_vjp<DIM>(p.inv_gm1, qa, qbn, nrm, g, qb[ia], qb[ib]);
       }
     }
-    // LLF faces: this element's share of each face flux's adjoint
+    // faces: this element's share of each face flux's adjoint
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll 1
     for (int side = 0; side < 2; ++side) {
       const bool lower = side == 0;
       const int64_t nb = p.nbr[e * 2 * DIM + 2 * dir + side];
+      if (DIM == 2 && nb < 0) {
+        // a non-periodic face (2D meshes only): the flux enters this element
+        // alone, and the outer state's cotangent maps back through the
+        // boundary condition
+        const int od = lower ? dofs[0] : dofs[n - 1];
+        double uo[NV], ro[NV], uob[NV], ul[NV], ur[NV], nrm[DIM], g[NV], ubl[NV], ubr[NV];
+        load_state<DIM, NN>(p.u, e, od, uo);
+        load_r<DIM, NN, NODAL>(a, e, od, ro);
+        ja(e, od, nrm);
+        const double sg = sign_j<NODAL, DIM, NN>(p, e, od);
+        for (int c = 0; c < DIM; ++c) nrm[c] *= sg;
+        const int face = 2 * dir + side;
+        const int64_t bnode = boundary_node<DIM, N>(p, e, dir, line);
+        const BcMap m = bc_map<DIM>(p, face, bnode);
+        bc_outer_state<DIM, N, NN>(p, face, bnode, m, p.u + e * NV * NN, dofs, uo, nrm, uob);
+        for (int v = 0; v < NV; ++v) {
+          g[v] = lower ? -p.lhat0 * ro[v] : p.lhatN * ro[v];
+          ul[v] = lower ? uob[v] : uo[v];
+          ur[v] = lower ? uo[v] : uob[v];
+          ubl[v] = ubr[v] = 0.0;
+        }
+        if constexpr (SF == kFluxHLLC) {
+          hllc_vjp<DIM>(p.gamma, ul, ur, nrm, g, true, true, ubl, ubr);
+        } else {
+          const NodePrims<DIM> ql = node_prims<DIM>(p.gamma, ul), qr = node_prims<DIM>(p.gamma, ur);
+          llf_vjp_side<DIM>(p.gamma, ul, ql, ur, qr, nrm, g, true, ubl);
+          llf_vjp_side<DIM>(p.gamma, ul, ql, ur, qr, nrm, g, false, ubr);
+        }
+        double* own = lower ? ub_lo : ub_hi;
+        const double* outer = lower ? ubl : ubr;
+        for (int v = 0; v < NV; ++v) own[v] = lower ? ubr[v] : ubl[v];
+        if (m == BcMap::kMirror) {
+          // the mirror is symmetric: its transpose is itself
+          double mn = 0.0, nn = 0.0;
+          for (int c = 0; c < DIM; ++c) {
+            mn += outer[1 + c] * nrm[c];
+            nn += nrm[c] * nrm[c];
+          }
+          mn = mn / nn;
+          own[0] += outer[0];
+          for (int c = 0; c < DIM; ++c) own[1 + c] += outer[1 + c] - 2.0 * mn * nrm[c];
+          own[NV - 1] += outer[NV - 1];
+        } else if (m == BcMap::kMean) {
+          line_mean = true;
+          for (int v = 0; v < NV; ++v) ub_line[v] += outer[v] / (double)n;
+        }
+        continue;
+      }
       const int64_t el = lower ? nb : e, er = lower ? e : nb;
       const int dl = dofs[n - 1], dr = dofs[0];
       double ul[NV], ur[NV], nrm[DIM], g[NV], rl[NV], rrv[NV];
@@ -717,6 +886,10 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(c
       for (int v = 0; v < NV; ++v) {
         s_ub[(slot * NV + v) * NN + dofs[0]] += ub_lo[v];
         s_ub[(slot * NV + v) * NN + dofs[n - 1]] += ub_hi[v];
+      }
+      if (line_mean) {
+        for (int k = 0; k < n; ++k)
+          for (int v = 0; v < NV; ++v) s_ub[(slot * NV + v) * NN + dofs[k]] += ub_line[v];
       }
       // subcell fluxes of a troubled element: flux fi enters node fi - 1
       // with alpha w_inv(fi - 1) and node fi with -alpha w_inv(fi); each line

@@ -145,12 +145,10 @@ def test_boundary_ssprk3_steps(gpu, layout):
 
 
 def test_boundary_configurations_rejected(gpu):
-    """Batches and the tangent/adjoint are built for periodic meshes; they
-    refuse non-periodic cases instead of reading missing neighbours."""
+    """Batches are built for periodic meshes; they refuse non-periodic cases
+    instead of reading missing neighbours.  (The Euler tangent and adjoint
+    take them: test_gpu_adjoint_bc.py.)"""
     _, gs = make_bc_pair(2, (4, 4), LAYOUTS["walls"])
-    u = gs.empty().fill_(1.0)
-    with pytest.raises(dg.DgsemError):
-        gs.rhs_jvp(u, u)
     cfg = gs.cfg
     with pytest.raises(dg.DgsemError):
         dg.Batch([cfg, cfg])
