@@ -202,8 +202,6 @@ int ad_prepare(dgsem_handle* h) {
   const bool sc = c.volume_integral == DGSEM_VOLUME_SHOCK_CAPTURING;
   if (dgsem::uses_central_flux(c))
     return afail(h, DGSEM_E_UNSUPPORTED, "CentralFlux is not differentiated");
-  if (c.positivity_floor > 0.0)
-    return afail(h, DGSEM_E_UNSUPPORTED, "the positivity floor is not differentiated");
   if (c.equation == DGSEM_NAVIER_STOKES)
     for (int f = 0; f < 2 * c.dim; ++f)
       if (c.face_boundary[f] != DGSEM_BC_PERIODIC)
@@ -282,6 +280,13 @@ int apply(dgsem_handle* h, bool adjoint, const double* u, const double* v, const
 
 bool distinct(const void* a, const void* b) { return a && b && a != b; }
 
+// The positivity floor clamps the stage results (the RHS itself is the same
+// with or without it), so only the step derivatives refuse it.
+int floor_refused(dgsem_handle* h) {
+  return afail(h, DGSEM_E_UNSUPPORTED,
+               "the positivity floor (a clamp of the stage results) is not differentiated");
+}
+
 }  // namespace
 
 namespace dgsem {
@@ -328,6 +333,7 @@ int dgsem_ssprk3_step_jvp(dgsem_handle* h, double dt, const double* u0, const do
                           const double* u2, const double* du0, double* du_out, double* tmp1,
                           double* tmp2) {
   if (!h || !h->d_Ja) return DGSEM_E_INVALID;
+  if (h->cfg.positivity_floor > 0.0) return floor_refused(h);
   if (!u0 || !u1 || !u2 || !du0 || !du_out || !tmp1 || !tmp2 || tmp1 == tmp2 || du_out == tmp1 ||
       du_out == tmp2 || du_out == du0)
     return afail(h, DGSEM_E_INVALID, "du0, du_out, tmp1, tmp2 must be distinct device arrays");
@@ -345,6 +351,7 @@ int dgsem_ssprk3_step_vjp(dgsem_handle* h, double dt, const double* u0, const do
                           const double* u2, const double* ubar, double* ubar_out, double* tmp1,
                           double* tmp2) {
   if (!h || !h->d_Ja) return DGSEM_E_INVALID;
+  if (h->cfg.positivity_floor > 0.0) return floor_refused(h);
   if (!u0 || !u1 || !u2 || !ubar || !ubar_out || !tmp1 || !tmp2 || tmp1 == tmp2 ||
       ubar == tmp1 || ubar == tmp2 || ubar_out == tmp1 || ubar_out == tmp2)
     return afail(h, DGSEM_E_INVALID, "ubar, tmp1, tmp2 must be distinct device arrays");

@@ -247,18 +247,25 @@ def test_checkpointed_khi_gradient_matches_frozen_finite_differences(gpu, amax):
     assert worst <= 1e-6
 
 
-def test_refused_configurations(gpu):
-    """The positivity floor is not differentiated: the derivative entry
-    points return DGSEM_E_UNSUPPORTED and leave the output untouched."""
+def test_positivity_floor(gpu):
+    """The positivity floor clamps the stage results; the RHS is the same
+    with or without it, and so are its tangent and adjoint.  The step
+    derivatives, which would pass through the clamp, return
+    DGSEM_E_UNSUPPORTED and leave the output untouched."""
     kw = dict(dim=2, polydeg=3, cells=(4, 4), domain_left=(-1, -1), domain_right=(1, 1),
-              volume_integral=dg.SHOCK_CAPTURING, positivity_floor=1e-10)
-    gs = dg.StructuredSolver(dg.SolverConfig(**kw))
+              volume_integral=dg.SHOCK_CAPTURING)
+    plain = dg.StructuredSolver(dg.SolverConfig(**kw))
+    gs = dg.StructuredSolver(dg.SolverConfig(positivity_floor=1e-10, **kw))
     u = gs.from_reference_layout(perturbed_khi(gs.node_coordinates()))
-    v = torch.ones_like(u)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    v = torch.randn(gs.shape, dtype=torch.float64, device="cuda", generator=g)
+    assert torch.equal(gs.rhs_jvp(u, v), plain.rhs_jvp(u, v))
+    assert torch.equal(gs.rhs_vjp(u, v), plain.rhs_vjp(u, v))
+    u1, u2 = plain.ssprk3_stages(u, 1e-4)
     out = torch.full_like(u, 7.0)
     with pytest.raises(dg.DgsemError, match="not differentiated"):
-        gs.rhs_jvp(u, v, out)
+        gs.step_jvp(1e-4, u, u1, u2, v, out)
     with pytest.raises(dg.DgsemError, match="not differentiated"):
-        gs.rhs_vjp(u, v, out)
+        gs.step_vjp(1e-4, u, u1, u2, v, out)
     torch.cuda.synchronize()
     assert bool((out == 7.0).all())
