@@ -702,6 +702,18 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_visc2_kernel
   }
 }
 
+// Whether the adjoint kernel forms the subcell cotangents of all directions
+// at once: in 3D up to N = 3, where the per-direction arrays keep the LDS of
+// a block within 80 KiB, so the two blocks per CU its registers allow still
+// fit.  (With HLLC subcells and every element troubled, 3D N = 3 at 32^3,
+// the adjoint took 5.7 ms with one direction at a time.)
+template <int DIM, int N, int EPW, bool SC>
+constexpr bool par_subcell_adjoint() {
+  using S = Shape<DIM, N>;
+  constexpr int lds = 8 * EPW * S::NN * (S::NP + 2 * S::NV + DIM + 3 + DIM * S::NV);
+  return SC && DIM == 3 && N <= 3 && lds <= 80 * 1024;
+}
+
 template <int DIM, int N, int EPW, bool NODAL, bool NS, bool SC, int SF, int FF>
 __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(const AdParams a) {
   using S = Shape<DIM, N>;
@@ -713,6 +725,11 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(c
   __shared__ double s_r[EPW * NV * NN];
   __shared__ double s_qb[EPW * NQ * NN];
   __shared__ double s_ub[EPW * NV * NN];
+  // PAR_FV (shock capturing, where the LDS allows): the troubled elements'
+  // subcell cotangents are formed by all directions' lines at once, into
+  // one array per direction, instead of one direction at a time.
+  constexpr bool PAR_FV = par_subcell_adjoint<DIM, N, EPW, SC>();
+  __shared__ double s_fv[PAR_FV ? DIM * EPW * NV * NN : 1];
   const int tid = threadIdx.x;
   const int64_t e0 = (int64_t)blockIdx.x * EPW;
   const int nvalid = (int)((p.E - e0) < EPW ? (p.E - e0) : EPW);
@@ -899,7 +916,7 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(c
       // subcell fluxes of a troubled element: flux fi enters node fi - 1
       // with alpha w_inv(fi - 1) and node fi with -alpha w_inv(fi); each line
       // owns its nodes within this direction, so the sums need no atomics
-      if constexpr (SC) {
+      if constexpr (SC && !PAR_FV) {
         if (troubled) {
           const int64_t e = e0 + slot;
           const double* rr = s_r + slot * NV * NN;
@@ -931,6 +948,42 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(c
     }
     __syncthreads();
   }
+  // PAR_FV: the same subcell terms, all directions at once now that the
+  // line's other sums are in LDS.  Each direction has its own array, where
+  // the line owns its nodes: node fi - 1 gets its second term here, node fi
+  // its first.
+  if constexpr (PAR_FV) {
+    if (troubled) {
+      const int64_t e = e0 + slot;
+      const double* rr = s_r + slot * NV * NN;
+      double* fvp = s_fv + (dir * EPW + slot) * NV * NN;
+#pragma unroll 1
+      for (int fi = 1; fi < n; ++fi) {
+        double ul[NV], ur[NV], nrm[DIM], g[NV], ubl[NV], ubr[NV];
+        load_state<DIM, NN>(p.u, e, dofs[fi - 1], ul);
+        load_state<DIM, NN>(p.u, e, dofs[fi], ur);
+        subcell_normal<DIM, N, NODAL>(p, e, dir, line, fi - 1, nrm);
+        const double wl = alpha * p.inv_w[fi - 1], wr = alpha * p.inv_w[fi];
+        for (int v = 0; v < NV; ++v) {
+          g[v] = wl * rr[v * NN + dofs[fi - 1]] - wr * rr[v * NN + dofs[fi]];
+          ubl[v] = ubr[v] = 0.0;
+        }
+        if constexpr (FF == kFluxHLLC) {
+          hllc_vjp<DIM>(p.gamma, ul, ur, nrm, g, true, true, ubl, ubr);
+        } else {
+          const NodePrims<DIM> ql = node_prims<DIM>(p.gamma, ul), qr = node_prims<DIM>(p.gamma, ur);
+          llf_vjp_side<DIM>(p.gamma, ul, ql, ur, qr, nrm, g, true, ubl);
+          llf_vjp_side<DIM>(p.gamma, ul, ql, ur, qr, nrm, g, false, ubr);
+        }
+        for (int v = 0; v < NV; ++v) {
+          double* lo = fvp + v * NN + dofs[fi - 1];
+          *lo = fi == 1 ? ubl[v] : *lo + ubl[v];
+          fvp[v * NN + dofs[fi]] = ubr[v];
+        }
+      }
+    }
+    __syncthreads();
+  }
   for (int i = tid; i < EPW * NN; i += BLOCK) {
     const int sl = i / NN, node = i % NN;
     if (sl >= nvalid) continue;
@@ -951,6 +1004,13 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_rhs_kernel(c
     qbar.kin = t[(DIM + 2) * NN];
     double ub[NV];
     for (int v = 0; v < NV; ++v) ub[v] = s_ub[(sl * NV + v) * NN + node];
+    if constexpr (PAR_FV) {
+      double cf, al;
+      if (frozen_blend<SC>(p, e, cf, al)) {
+        for (int d = 0; d < DIM; ++d)
+          for (int v = 0; v < NV; ++v) ub[v] += s_fv[((d * EPW + sl) * NV + v) * NN + node];
+      }
+    }
     prims_vjp<DIM>(p.gamma, q, qbar, 0.0, ub);
     for (int v = 0; v < NV; ++v) {
       const int64_t g = (e * NV + v) * NN + node;
