@@ -351,10 +351,11 @@ DG_HD void llf_vjp_side(double gamma, const double* ul, const NodePrims<DIM>& ql
 }
 
 // ---------------------------------------------------------------------
-// HLLC: forward-mode dual numbers through the flux as physics.hpp writes it
-// (flux_hllc is templated on its scalar type), so the tangent follows the
-// branches (wave-speed bounds, star region, side) the forward evaluation
-// takes.  The adjoint builds the Jacobian's columns one input at a time.
+// HLLC: the tangent by forward-mode dual numbers through the flux as
+// physics.hpp writes it (flux_hllc is templated on its scalar type), so it
+// follows the branches (wave-speed bounds, star region, side) the forward
+// evaluation takes; the adjoint by hand in reverse mode, through the same
+// branches.
 // ---------------------------------------------------------------------
 
 // value and derivative; arithmetic by the usual rules, comparisons on values
@@ -402,12 +403,13 @@ DG_HD void hllc_jvp(double gamma, const double* ul, const double* ur, const doub
   for (int i = 0; i < NV; ++i) df[i] = f[i].d;
 }
 
-// ubar_side += (dF/du_side)^T g for the sides selected: one column of the
-// Jacobian per input component.
+// ubar_side += (dF/du_side)^T g for the sides selected, by forward-mode
+// dual numbers: one column of the Jacobian per input component.  The
+// kernels use the reverse-mode hllc_vjp below; this form serves as its check.
 template <int DIM>
-DG_HD void hllc_vjp(double gamma, const double* ul, const double* ur, const double* n,
-                    const double* g, bool with_left, bool with_right, double* ubar_l,
-                    double* ubar_r) {
+DG_HD void hllc_vjp_columns(double gamma, const double* ul, const double* ur, const double* n,
+                            const double* g, bool with_left, bool with_right, double* ubar_l,
+                            double* ubar_r) {
   constexpr int NV = DIM + 2;
   for (int side = 0; side < 2; ++side) {
     if (side == 0 ? !with_left : !with_right) continue;
@@ -424,6 +426,201 @@ DG_HD void hllc_vjp(double gamma, const double* ul, const double* ur, const doub
       ub[j] += s;
     }
   }
+}
+
+// ubar_side += (dF/du_side)^T g for the sides selected, in reverse mode: the
+// forward pass of flux_hllc (the same expressions, so the same wave-speed,
+// star-region and side choices), then the cotangents back through it, about
+// the cost of two flux evaluations where the dual-number columns cost ten.
+template <int DIM>
+DG_HD void hllc_vjp(double gamma, const double* ul, const double* ur, const double* n,
+                    const double* g, bool with_left, bool with_right, double* ubar_l,
+                    double* ubar_r) {
+  constexpr int NV = DIM + 2;
+  const double gm1 = gamma - 1.0;
+  // forward
+  double nrm = 0.0, nh[DIM], vl[DIM], vr[DIM];
+  for (int d = 0; d < DIM; ++d) nrm += n[d] * n[d];
+  nrm = sqrt(nrm);
+  for (int d = 0; d < DIM; ++d) nh[d] = n[d] / nrm;
+  const double rl = ul[0], rr = ur[0], El = ul[NV - 1], Er = ur[NV - 1];
+  double v2l = 0.0, v2r = 0.0, vnl = 0.0, vnr = 0.0;
+  for (int d = 0; d < DIM; ++d) {
+    vl[d] = ul[d + 1] / rl;
+    vr[d] = ur[d + 1] / rr;
+    v2l += vl[d] * vl[d];
+    v2r += vr[d] * vr[d];
+    vnl += vl[d] * nh[d];
+    vnr += vr[d] * nh[d];
+  }
+  const double el = El / rl, er = Er / rr;
+  const double pl = (gamma - 1.0) * (El - 0.5 * rl * v2l);
+  const double pr = (gamma - 1.0) * (Er - 0.5 * rr * v2r);
+  const double cl = sqrt(gamma * pl / rl), cr = sqrt(gamma * pr / rr);
+  const double sl = sqrt(rl), sr = sqrt(rr), ss = sl + sr;
+  double vd[DIM], vroe2 = 0.0;
+  for (int d = 0; d < DIM; ++d) {
+    vd[d] = (sl * vl[d] + sr * vr[d]) / ss;
+    vroe2 += vd[d] * vd[d];
+  }
+  const double vroe = (sl * vnl + sr * vnr) / ss;
+  const double Hl = (El + pl) / rl, Hr = (Er + pr) / rr;
+  const double Hroe = (sl * Hl + sr * Hr) / ss;
+  const double croe = sqrt((gamma - 1.0) * (Hroe - 0.5 * vroe2));
+  // SL = dmin(vnl - cl, vroe - croe), SR = dmax(vnr + cr, vroe + croe)
+  const double aL = vnl - cl, bL = vroe - croe, aR = vnr + cr, bR = vroe + croe;
+  const bool SL_roe = bL < aL, SR_roe = aR < bR;
+  const double SL = SL_roe ? bL : aL, SR = SR_roe ? bR : aR;
+  const double mL = SL - vnl, mR = SR - vnr;
+  const bool star = !(SL >= 0) && !(SR <= 0);
+  double Ss = 0.0, Den = 1.0;
+  if (star) {
+    Den = rl * mL - rr * mR;
+    Ss = (pr - pl + rl * vnl * mL - rr * vnr * mR) / (rl * mL - rr * mR);
+  }
+  const bool left = star ? Ss >= 0 : SL >= 0;
+  const double* uk = left ? ul : ur;
+
+  // reverse: f = nrm * (fk(uk) [+ Sk (star terms)])
+  double G[NV];
+  for (int i = 0; i < NV; ++i) G[i] = g[i] * nrm;
+  double ulb[NV], urb[NV], ukb[NV];
+  for (int i = 0; i < NV; ++i) ulb[i] = urb[i] = ukb[i] = 0.0;
+  flux_n_vjp<DIM>(gamma, uk, node_prims<DIM>(gamma, uk), nh, G, ukb);
+  double rl_b = 0.0, rr_b = 0.0, El_b = 0.0, Er_b = 0.0, vnl_b = 0.0, vnr_b = 0.0;
+  double pl_b = 0.0, pr_b = 0.0, el_b = 0.0, er_b = 0.0, vl_b[DIM], vr_b[DIM];
+  for (int d = 0; d < DIM; ++d) vl_b[d] = vr_b[d] = 0.0;
+  if (star) {
+    const double* vk = left ? vl : vr;
+    const double rk = left ? rl : rr, vnk = left ? vnl : vnr, ek = left ? el : er;
+    const double pk = left ? pl : pr, Sk = left ? SL : SR, mk = left ? mL : mR;
+    const double D = Sk - Ss, rs = rk * mk / D;
+    const double X = Ss - vnk, rkmk = rk * mk, Z = pk / rkmk, Y = Ss + Z, es = ek + X * Y;
+    // f0 = fk0 + Sk (rs - uk0), f_d = fk_d + Sk (rs (vk_d + X nh_d) - uk_d),
+    // fE = fkE + Sk (rs es - ukE)
+    double Sk_b = G[0] * (rs - uk[0]) + G[NV - 1] * (rs * es - uk[NV - 1]);
+    double rs_b = Sk * (G[0] + G[NV - 1] * es), gn = 0.0, vk_b[DIM];
+    for (int d = 0; d < DIM; ++d) {
+      const double w = vk[d] + X * nh[d];
+      Sk_b += G[d + 1] * (rs * w - uk[d + 1]);
+      rs_b += Sk * G[d + 1] * w;
+      vk_b[d] = Sk * rs * G[d + 1];
+      gn += G[d + 1] * nh[d];
+    }
+    for (int i = 0; i < NV; ++i) ukb[i] -= Sk * G[i];
+    const double es_b = Sk * rs * G[NV - 1];
+    double X_b = Sk * rs * gn + es_b * Y, Ss_b = 0.0;
+    const double Y_b = es_b * X, ek_b = es_b;
+    // Y = Ss + pk / (rk mk)
+    Ss_b += Y_b;
+    const double pk_b = Y_b / rkmk, rkmk_b = -Y_b * Z / rkmk;
+    // X = Ss - vnk
+    Ss_b += X_b;
+    double vnk_b = -X_b;
+    // rs = rk mk / (Sk - Ss)
+    const double rk_b = rs_b * mk / D + rkmk_b * mk;
+    double mk_b = rs_b * rk / D + rkmk_b * rk;
+    const double D_b = -rs_b * rs / D;
+    Sk_b += D_b;
+    Ss_b -= D_b;
+    double SL_b = 0.0, SR_b = 0.0, mL_b = 0.0, mR_b = 0.0;
+    if (left) {
+      rl_b += rk_b; vnl_b += vnk_b; el_b += ek_b; pl_b += pk_b; SL_b += Sk_b; mL_b += mk_b;
+      for (int d = 0; d < DIM; ++d) vl_b[d] += vk_b[d];
+    } else {
+      rr_b += rk_b; vnr_b += vnk_b; er_b += ek_b; pr_b += pk_b; SR_b += Sk_b; mR_b += mk_b;
+      for (int d = 0; d < DIM; ++d) vr_b[d] += vk_b[d];
+    }
+    // Ss = (pr - pl + rl vnl mL - rr vnr mR) / (rl mL - rr mR)
+    const double Num_b = Ss_b / Den, Den_b = -Ss_b * Ss / Den;
+    pr_b += Num_b;
+    pl_b -= Num_b;
+    rl_b += Num_b * vnl * mL + Den_b * mL;
+    vnl_b += Num_b * rl * mL;
+    mL_b += Num_b * rl * vnl + Den_b * rl;
+    rr_b -= Num_b * vnr * mR + Den_b * mR;
+    vnr_b -= Num_b * rr * mR;
+    mR_b -= Num_b * rr * vnr + Den_b * rr;
+    // mL = SL - vnl, mR = SR - vnr
+    SL_b += mL_b;
+    vnl_b -= mL_b;
+    SR_b += mR_b;
+    vnr_b -= mR_b;
+    // the wave-speed bounds, through the branch the forward took
+    double vroe_b = 0.0, croe_b = 0.0, cl_b = 0.0, cr_b = 0.0;
+    if (SL_roe) {
+      vroe_b += SL_b;
+      croe_b -= SL_b;
+    } else {
+      vnl_b += SL_b;
+      cl_b -= SL_b;
+    }
+    if (SR_roe) {
+      vroe_b += SR_b;
+      croe_b += SR_b;
+    } else {
+      vnr_b += SR_b;
+      cr_b += SR_b;
+    }
+    // croe = sqrt((gamma - 1)(Hroe - vroe2 / 2))
+    const double W_b = croe_b / (2.0 * croe);
+    const double Hroe_b = gm1 * W_b, vroe2_b = -0.5 * gm1 * W_b;
+    double sl_b = Hroe_b * (Hl - Hroe) / ss + vroe_b * (vnl - vroe) / ss;
+    double sr_b = Hroe_b * (Hr - Hroe) / ss + vroe_b * (vnr - vroe) / ss;
+    const double Hl_b = Hroe_b * sl / ss, Hr_b = Hroe_b * sr / ss;
+    vnl_b += vroe_b * sl / ss;
+    vnr_b += vroe_b * sr / ss;
+    for (int d = 0; d < DIM; ++d) {
+      const double vd_b = 2.0 * vd[d] * vroe2_b;
+      vl_b[d] += vd_b * sl / ss;
+      vr_b[d] += vd_b * sr / ss;
+      sl_b += vd_b * (vl[d] - vd[d]) / ss;
+      sr_b += vd_b * (vr[d] - vd[d]) / ss;
+    }
+    // H = (E + p) / rho, s = sqrt(rho), c = sqrt(gamma p / rho)
+    El_b += Hl_b / rl;
+    pl_b += Hl_b / rl;
+    rl_b -= Hl_b * Hl / rl;
+    Er_b += Hr_b / rr;
+    pr_b += Hr_b / rr;
+    rr_b -= Hr_b * Hr / rr;
+    rl_b += sl_b / (2.0 * sl);
+    rr_b += sr_b / (2.0 * sr);
+    if (cl_b != 0.0) {
+      const double Q_b = cl_b / (2.0 * cl);
+      pl_b += Q_b * gamma / rl;
+      rl_b -= Q_b * cl * cl / rl;
+    }
+    if (cr_b != 0.0) {
+      const double Q_b = cr_b / (2.0 * cr);
+      pr_b += Q_b * gamma / rr;
+      rr_b -= Q_b * cr * cr / rr;
+    }
+  }
+  // e = E / rho, p = (gamma - 1)(E - rho |v|^2 / 2), v = m / rho
+  El_b += el_b / rl + gm1 * pl_b;
+  rl_b -= el_b * el / rl + 0.5 * gm1 * v2l * pl_b;
+  Er_b += er_b / rr + gm1 * pr_b;
+  rr_b -= er_b * er / rr + 0.5 * gm1 * v2r * pr_b;
+  const double v2l_b = -0.5 * gm1 * rl * pl_b, v2r_b = -0.5 * gm1 * rr * pr_b;
+  for (int d = 0; d < DIM; ++d) {
+    vl_b[d] += 2.0 * vl[d] * v2l_b + vnl_b * nh[d];
+    vr_b[d] += 2.0 * vr[d] * v2r_b + vnr_b * nh[d];
+    ulb[d + 1] += vl_b[d] / rl;
+    rl_b -= vl_b[d] * vl[d] / rl;
+    urb[d + 1] += vr_b[d] / rr;
+    rr_b -= vr_b[d] * vr[d] / rr;
+  }
+  ulb[0] += rl_b;
+  ulb[NV - 1] += El_b;
+  urb[0] += rr_b;
+  urb[NV - 1] += Er_b;
+  double* ukside = left ? ulb : urb;
+  for (int i = 0; i < NV; ++i) ukside[i] += ukb[i];
+  if (with_left)
+    for (int i = 0; i < NV; ++i) ubar_l[i] += ulb[i];
+  if (with_right)
+    for (int i = 0; i < NV; ++i) ubar_r[i] += urb[i];
 }
 
 // ---------------------------------------------------------------------

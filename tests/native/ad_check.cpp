@@ -169,7 +169,7 @@ void check_llf() {
 template <int DIM>
 void check_hllc() {
   constexpr int NV = DIM + 2;
-  double worst_fd = 0.0, worst_dot = 0.0;
+  double worst_fd = 0.0, worst_dot = 0.0, worst_cols = 0.0;
   for (int trial = 0; trial < 400; ++trial) {
     double ul[NV], ur[NV], dl[NV], dr[NV], n[DIM], g[NV];
     random_state<DIM>(ul);
@@ -210,8 +210,14 @@ void check_hllc() {
     for (int i = 0; i < NV; ++i) scale = std::fmax(scale, std::fabs(df[i]));
     for (int i = 0; i < NV; ++i)
       worst_fd = std::fmax(worst_fd, rel((fp[i] - fm[i]) / (2 * h), df[i], scale));
-    double bl[NV] = {}, br[NV] = {};
+    double bl[NV] = {}, br[NV] = {}, cl[NV] = {}, cr[NV] = {};
     hllc_vjp<DIM>(1.4, ul, ur, n, g, true, true, bl, br);
+    // the reverse-mode adjoint against the dual-number columns
+    hllc_vjp_columns<DIM>(1.4, ul, ur, n, g, true, true, cl, cr);
+    double cmag = 1e-300;
+    for (int i = 0; i < NV; ++i) cmag = std::fmax(cmag, std::fmax(std::fabs(cl[i]), std::fabs(cr[i])));
+    for (int i = 0; i < NV; ++i)
+      worst_cols = std::fmax(worst_cols, std::fmax(std::fabs(bl[i] - cl[i]), std::fabs(br[i] - cr[i])) / cmag);
     double lhs = 0.0, rhs = 0.0, mag = 0.0;
     for (int i = 0; i < NV; ++i) {
       lhs += g[i] * df[i];
@@ -223,6 +229,8 @@ void check_hllc() {
   char name[64];
   std::snprintf(name, sizeof(name), "hllc%dD", DIM);
   report(name, worst_fd, worst_dot, 1e-6, 1e-13);
+  std::snprintf(name, sizeof(name), "hllc%dD reverse vs columns", DIM);
+  report(name, 0.0, worst_cols, 1e-6, 1e-13);
 }
 
 template <int DIM>

@@ -2,8 +2,9 @@
 (dgsem_amd/csrc/physics_ad.hpp), compiled for the CPU: tangent against
 central finite differences, adjoint against tangent by the dot-product
 identity, for the Chandrashekar flux (both branches of the logarithmic mean),
-the LLF flux, the HLLC flux (dual numbers) and the Navier-Stokes viscous
-flux (tests/native/ad_check.cpp)."""
+the LLF flux, the HLLC flux (tangent by dual numbers, adjoint in reverse
+mode and against the dual-number Jacobian columns) and the Navier-Stokes
+viscous flux (tests/native/ad_check.cpp)."""
 import os
 import subprocess
 
@@ -17,4 +18,4 @@ def test_pointwise_derivatives(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True)
     print(r.stdout)
     assert r.returncode == 0, r.stdout
-    assert r.stdout.count(" ok") == 10
+    assert r.stdout.count(" ok") == 12
