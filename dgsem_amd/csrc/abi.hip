@@ -76,8 +76,10 @@ void release(dgsem_handle* h) {
   hipFree(h->d_an_partial);
   hipFree(h->d_ad_alpha);
   hipFree(h->d_ad_out);
+  hipFree(h->d_fl_r);
+  hipFree(h->d_fl_w);
   h->d_ad_dgrad = h->d_ad_dvisc = h->d_ad_qbar = h->d_wq = h->d_ke_partial = nullptr;
-  h->d_an_partial = h->d_ad_alpha = h->d_ad_out = nullptr;
+  h->d_an_partial = h->d_ad_alpha = h->d_ad_out = h->d_fl_r = h->d_fl_w = nullptr;
   h->d_Ja = h->d_invJ = h->d_sn = h->d_Vinv = h->d_alpha = h->d_grad = h->d_visc = nullptr;
   h->d_nbr = nullptr;
   h->d_ja_box = nullptr;

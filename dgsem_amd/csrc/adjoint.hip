@@ -280,11 +280,86 @@ int apply(dgsem_handle* h, bool adjoint, const double* u, const double* v, const
 
 bool distinct(const void* a, const void* b) { return a && b && a != b; }
 
-// The positivity floor clamps the stage results (the RHS itself is the same
-// with or without it), so only the step derivatives refuse it.
-int floor_refused(dgsem_handle* h) {
-  return afail(h, DGSEM_E_UNSUPPORTED,
-               "the positivity floor (a clamp of the stage results) is not differentiated");
+// The positivity floor in the step derivatives.  A stage's result is the
+// stage combination pre = c1 u_base + c2 u_k + c3 R(u_k) (stage 1:
+// c1 u_k + c2 R(u_k)) clamped as the forward kernel's epilogue clamps it:
+// rho' = max(rho, floor), then E' = floor / (gamma - 1) + kin(rho', m) where
+// the pressure would fall below the floor.  This applies the clamp's
+// derivative at pre to w in place: the tangent (adjoint = false) or its
+// transpose.  pre is formed with the forward's expressions from R(u_k), so
+// the clamps are the forward's.
+template <int DIM>
+__global__ void floor_clamp_kernel(int64_t nodes, int nn, const double* __restrict__ u_base,
+                                   const double* __restrict__ u_k, const double* __restrict__ r,
+                                   double c1, double c2, double c3, double floor_value, double gamma,
+                                   bool adjoint, double* __restrict__ w) {
+  constexpr int NV = DIM + 2;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nodes) return;
+  const int64_t g = (i / nn) * NV * nn + i % nn;
+  double s[NV], x[NV];
+  for (int v = 0; v < NV; ++v) {
+    const double uk = u_k[g + v * nn];
+    s[v] = u_base ? c1 * u_base[g + v * nn] + c2 * uk + c3 * r[g + v * nn] : c1 * uk + c2 * r[g + v * nn];
+    x[v] = w[g + v * nn];
+  }
+  const bool rho_clamped = s[0] < floor_value;
+  const double rho = rho_clamped ? floor_value : s[0];
+  double m2 = 0.0;
+  for (int k = 0; k < DIM; ++k) m2 += s[k + 1] * s[k + 1];
+  const double kin = 0.5 * m2 / rho;
+  const bool e_clamped = (gamma - 1.0) * (s[NV - 1] - kin) < floor_value;
+  if (!adjoint) {
+    const double drho = rho_clamped ? 0.0 : x[0];
+    if (e_clamped) {
+      double mdm = 0.0;
+      for (int k = 0; k < DIM; ++k) mdm += s[k + 1] * x[k + 1];
+      x[NV - 1] = mdm / rho - kin / rho * drho;
+    }
+    x[0] = drho;
+  } else {
+    double rho_b = x[0];
+    if (e_clamped) {
+      const double kin_b = x[NV - 1];
+      for (int k = 0; k < DIM; ++k) x[k + 1] += kin_b * s[k + 1] / rho;
+      rho_b -= kin_b * kin / rho;
+      x[NV - 1] = 0.0;
+    }
+    x[0] = rho_clamped ? 0.0 : rho_b;
+  }
+  for (int v = 0; v < NV; ++v) w[g + v * nn] = x[v];
+}
+
+// Applies the derivative of stage `stage`'s clamp (SSPRK3 coefficients as
+// stage_params forms them) to w in place; u_base is u0 for stages 2 and 3.
+int floor_stage(dgsem_handle* h, bool adjoint, int stage, double dt, const double* u_base,
+                const double* u_k, double* w) {
+  const size_t cell = (size_t)h->E * h->nvars * h->nn;
+  int rc;
+  if ((rc = alloc_once(h, &h->d_fl_r, cell)) || (rc = alloc_once(h, &h->d_ad_alpha, (size_t)h->E)))
+    return rc;
+  // R(u_k), by the forward kernel with the blending factor the forward step
+  // used at u_k (the indicator's, or the frozen array)
+  dgsem::KParams f = h->base;
+  f.u = u_k;
+  f.out = h->d_fl_r;
+  f.alpha = h->d_ad_alpha;
+  f.mode = dgsem::kModeRhs;
+  if ((rc = dgsem::launch_params(h, f))) return rc;
+  dgsem::KParams sp;
+  if ((rc = dgsem::stage_params(h, stage, dt, stage == 1 ? nullptr : u_base, u_k, w, &sp))) return rc;
+  const int64_t nodes = h->E * (int64_t)h->nn;
+  const unsigned grid = (unsigned)((nodes + 255) / 256);
+  if (h->cfg.dim == 2)
+    hipLaunchKernelGGL(floor_clamp_kernel<2>, dim3(grid), dim3(256), 0, h->stream, nodes, h->nn,
+                       sp.u_base, u_k, h->d_fl_r, sp.c1, sp.c2, sp.c3, h->cfg.positivity_floor,
+                       h->cfg.gamma, adjoint, w);
+  else
+    hipLaunchKernelGGL(floor_clamp_kernel<3>, dim3(grid), dim3(256), 0, h->stream, nodes, h->nn,
+                       sp.u_base, u_k, h->d_fl_r, sp.c1, sp.c2, sp.c3, h->cfg.positivity_floor,
+                       h->cfg.gamma, adjoint, w);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? DGSEM_OK : ahip(h, e, "floor clamp kernel launch");
 }
 
 }  // namespace
@@ -333,37 +408,53 @@ int dgsem_ssprk3_step_jvp(dgsem_handle* h, double dt, const double* u0, const do
                           const double* u2, const double* du0, double* du_out, double* tmp1,
                           double* tmp2) {
   if (!h || !h->d_Ja) return DGSEM_E_INVALID;
-  if (h->cfg.positivity_floor > 0.0) return floor_refused(h);
   if (!u0 || !u1 || !u2 || !du0 || !du_out || !tmp1 || !tmp2 || tmp1 == tmp2 || du_out == tmp1 ||
       du_out == tmp2 || du_out == du0)
     return afail(h, DGSEM_E_INVALID, "du0, du_out, tmp1, tmp2 must be distinct device arrays");
+  const bool fl = h->cfg.positivity_floor > 0.0;
   int rc;
-  // du1 = du0 + dt J(u0) du0
+  // du1 = du0 + dt J(u0) du0 (then the clamp's derivative, with the floor)
   if ((rc = apply(h, false, u0, du0, nullptr, 0.0, nullptr, 0.0, 1.0, dt, tmp1))) return rc;
+  if (fl && (rc = floor_stage(h, false, 1, dt, nullptr, u0, tmp1))) return rc;
   // du2 = 3/4 du0 + 1/4 du1 + dt/4 J(u1) du1
   if ((rc = apply(h, false, u1, tmp1, du0, 3.0 / 4.0, nullptr, 0.0, 1.0 / 4.0, 1.0 / 4.0 * dt, tmp2)))
     return rc;
+  if (fl && (rc = floor_stage(h, false, 2, dt, u0, u1, tmp2))) return rc;
   // du3 = 1/3 du0 + 2/3 du2 + 2 dt/3 J(u2) du2
-  return apply(h, false, u2, tmp2, du0, 1.0 / 3.0, nullptr, 0.0, 2.0 / 3.0, 2.0 / 3.0 * dt, du_out);
+  if ((rc = apply(h, false, u2, tmp2, du0, 1.0 / 3.0, nullptr, 0.0, 2.0 / 3.0, 2.0 / 3.0 * dt, du_out)))
+    return rc;
+  return fl ? floor_stage(h, false, 3, dt, u0, u2, du_out) : DGSEM_OK;
 }
 
 int dgsem_ssprk3_step_vjp(dgsem_handle* h, double dt, const double* u0, const double* u1,
                           const double* u2, const double* ubar, double* ubar_out, double* tmp1,
                           double* tmp2) {
   if (!h || !h->d_Ja) return DGSEM_E_INVALID;
-  if (h->cfg.positivity_floor > 0.0) return floor_refused(h);
   if (!u0 || !u1 || !u2 || !ubar || !ubar_out || !tmp1 || !tmp2 || tmp1 == tmp2 ||
       ubar == tmp1 || ubar == tmp2 || ubar_out == tmp1 || ubar_out == tmp2)
     return afail(h, DGSEM_E_INVALID, "ubar, tmp1, tmp2 must be distinct device arrays");
   int rc;
-  // ubar2 = 2/3 ubar + 2 dt/3 J(u2)^T ubar
-  if ((rc = apply(h, true, u2, ubar, nullptr, 0.0, nullptr, 0.0, 2.0 / 3.0, 2.0 / 3.0 * dt, tmp1)))
+  // with the floor, each stage's clamp is transposed first: w3 (the last
+  // clamp's) in scratch, since ubar_out may alias ubar
+  const double* w3 = ubar;
+  if (h->cfg.positivity_floor > 0.0) {
+    if ((rc = alloc_once(h, &h->d_fl_w, (size_t)h->E * h->nvars * h->nn))) return rc;
+    const hipError_t e = hipMemcpyAsync(h->d_fl_w, ubar, sizeof(double) * h->E * h->nvars * h->nn,
+                                        hipMemcpyDeviceToDevice, h->stream);
+    if (e != hipSuccess) return ahip(h, e, "copy");
+    if ((rc = floor_stage(h, true, 3, dt, u0, u2, h->d_fl_w))) return rc;
+    w3 = h->d_fl_w;
+  }
+  // ubar2 = 2/3 w3 + 2 dt/3 J(u2)^T w3
+  if ((rc = apply(h, true, u2, w3, nullptr, 0.0, nullptr, 0.0, 2.0 / 3.0, 2.0 / 3.0 * dt, tmp1)))
     return rc;
+  if (w3 != ubar && (rc = floor_stage(h, true, 2, dt, u0, u1, tmp1))) return rc;
   // ubar1 = 1/4 ubar2 + dt/4 J(u1)^T ubar2
   if ((rc = apply(h, true, u1, tmp1, nullptr, 0.0, nullptr, 0.0, 1.0 / 4.0, 1.0 / 4.0 * dt, tmp2)))
     return rc;
-  // ubar0 = 1/3 ubar + 3/4 ubar2 + ubar1 + dt J(u0)^T ubar1  (ubar_out may alias ubar)
-  return apply(h, true, u0, tmp2, ubar, 1.0 / 3.0, tmp1, 3.0 / 4.0, 1.0, dt, ubar_out);
+  if (w3 != ubar && (rc = floor_stage(h, true, 1, dt, nullptr, u0, tmp2))) return rc;
+  // ubar0 = 1/3 w3 + 3/4 ubar2 + ubar1 + dt J(u0)^T ubar1  (ubar_out may alias ubar)
+  return apply(h, true, u0, tmp2, w3, 1.0 / 3.0, tmp1, 3.0 / 4.0, 1.0, dt, ubar_out);
 }
 
 int dgsem_kinetic_energy(dgsem_handle* h, const double* u, double* result) {

@@ -47,6 +47,10 @@ struct dgsem_handle {
   // state [E] and the forward RHS that computes it (discarded)
   double* d_ad_alpha = nullptr;
   double* d_ad_out = nullptr;
+  // step derivatives with the positivity floor: a stage input's RHS and the
+  // cotangent of the step's result after the last clamp [E][NV][NN]
+  double* d_fl_r = nullptr;
+  double* d_fl_w = nullptr;
   std::string err;
 };
 

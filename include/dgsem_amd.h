@@ -241,9 +241,10 @@ int dgsem_synchronize(dgsem_handle* h);
  * blending factor is not differentiated: it is held at the value the
  * forward evaluation at the same state computes (the indicator at u; at each
  * stage input for the step functions), or at the array set with
- * dgsem_freeze_alpha.  Navier-Stokes on bounded meshes is not
- * differentiated, and the step functions refuse the positivity floor, a
- * clamp of the stage results (DGSEM_E_UNSUPPORTED).
+ * dgsem_freeze_alpha.  With the positivity floor the step functions pass
+ * through each stage's clamp (its derivative at the stage's unclamped
+ * result).  Navier-Stokes on bounded meshes is not differentiated
+ * (DGSEM_E_UNSUPPORTED).
  * --------------------------------------------------------------------- */
 /* out = J(u) du */
 int dgsem_rhs_jvp(dgsem_handle* h, const double* u, const double* du, double* out);

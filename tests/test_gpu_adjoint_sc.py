@@ -247,11 +247,9 @@ def test_checkpointed_khi_gradient_matches_frozen_finite_differences(gpu, amax):
     assert worst <= 1e-6
 
 
-def test_positivity_floor(gpu):
+def test_positivity_floor_rhs(gpu):
     """The positivity floor clamps the stage results; the RHS is the same
-    with or without it, and so are its tangent and adjoint.  The step
-    derivatives, which would pass through the clamp, return
-    DGSEM_E_UNSUPPORTED and leave the output untouched."""
+    with or without it, and so are its tangent and adjoint."""
     kw = dict(dim=2, polydeg=3, cells=(4, 4), domain_left=(-1, -1), domain_right=(1, 1),
               volume_integral=dg.SHOCK_CAPTURING)
     plain = dg.StructuredSolver(dg.SolverConfig(**kw))
@@ -261,11 +259,70 @@ def test_positivity_floor(gpu):
     v = torch.randn(gs.shape, dtype=torch.float64, device="cuda", generator=g)
     assert torch.equal(gs.rhs_jvp(u, v), plain.rhs_jvp(u, v))
     assert torch.equal(gs.rhs_vjp(u, v), plain.rhs_vjp(u, v))
-    u1, u2 = plain.ssprk3_stages(u, 1e-4)
-    out = torch.full_like(u, 7.0)
-    with pytest.raises(dg.DgsemError, match="not differentiated"):
-        gs.step_jvp(1e-4, u, u1, u2, v, out)
-    with pytest.raises(dg.DgsemError, match="not differentiated"):
-        gs.step_vjp(1e-4, u, u1, u2, v, out)
-    torch.cuda.synchronize()
-    assert bool((out == 7.0).all())
+
+
+FLOOR_CASES = [
+    # the pressure clamp: the drifting bump with its pressure scaled to
+    # 0.1 .. 0.4 and the floor at 0.15 (density about 1); the density clamp:
+    # the Kelvin-Helmholtz state (rho 0.5 .. 1.25) with the floor at 0.6.
+    # (The Sedov state is at rest: |v.n| and the HLLC contact speed have
+    # their kinks at it.)
+    ("bump-pressure", 3, 3, (4, 4, 4), dg.FLUX_HLLC, 0.15),
+    ("khi-density", 2, 3, (8, 8), dg.FLUX_LLF, 0.6),
+]
+
+
+def low_pressure_bump(x, gamma=1.4):
+    """drifting_bump with the pressure 0.1 (1 + 3 g) in place of 1 + 3 g."""
+    u = drifting_bump(x, gamma)
+    kin = 0.5 * np.sum(u[..., 1:4] ** 2, axis=-1) / u[..., 0]
+    p = (gamma - 1) * (u[..., 4] - kin)
+    u[..., 4] = 0.1 * p / (gamma - 1) + kin
+    return u
+
+
+@pytest.mark.parametrize("case", FLOOR_CASES, ids=[c[0] for c in FLOOR_CASES])
+def test_positivity_floor_step(gpu, case):
+    """The step derivatives through the clamp of each stage's result: the
+    tangent against central differences of the GPU step (h = 1e-7 along a
+    direction scaled like the state at each node; the clamps act by a wide
+    margin at their nodes), and the tangent against the adjoint by the
+    dot-product identity."""
+    name, dim, N, cells, fv, floor = case
+    print(f"case floor step {name}", flush=True)
+    box = ([-1.0] * dim, [1.0] * dim)
+    kw = dict(dim=dim, polydeg=N, cells=cells, domain_left=box[0], domain_right=box[1],
+              volume_integral=dg.SHOCK_CAPTURING, surface_flux=fv, fv_flux=fv, alpha_max=0.5,
+              alpha_min=0.001)
+    gs = dg.StructuredSolver(dg.SolverConfig(positivity_floor=floor, **kw))
+    plain = dg.StructuredSolver(dg.SolverConfig(**kw))
+    x = gs.node_coordinates()
+    u0 = low_pressure_bump(x) if dim == 3 else perturbed_khi(x)
+    dt = 2e-3
+    u = gs.from_reference_layout(u0)
+    # the clamp acts: one step with and without the floor differ
+    a, b = u.clone(), u.clone()
+    t1, t2 = gs.empty(), gs.empty()
+    gs.ssprk3_steps(a, dt, 1, t1, t2)
+    plain.ssprk3_steps(b, dt, 1, t1, t2)
+    assert not torch.equal(a, b)
+    u1, u2 = gs.ssprk3_stages(u, dt)
+    # a direction scaled like the state node by node, so that the difference
+    # quotient perturbs every node relatively (and no clamp margin is crossed)
+    v = gs.from_reference_layout(direction(u0, 13))
+    g = torch.Generator(device="cuda").manual_seed(13)
+    w = torch.randn(gs.shape, dtype=torch.float64, device="cuda", generator=g)
+    jv = gs.step_jvp(dt, u, u1, u2, v)
+    h = 1e-7
+    outs = []
+    for sign in (1, -1):
+        y = (u + sign * h * v).contiguous()
+        gs.ssprk3_steps(y, dt, 1, t1, t2)
+        outs.append(y)
+    fd = (outs[0] - outs[1]) / (2 * h)
+    err = (torch.linalg.norm(fd - jv) / torch.linalg.norm(jv)).item()
+    print(f"  jvp vs fd rel {err:.2e}", flush=True)
+    assert err <= 1e-5
+    jtw = gs.step_vjp(dt, u, u1, u2, w)
+    lhs, rhs = torch.sum(w * jv).item(), torch.sum(jtw * v).item()
+    assert abs(lhs - rhs) <= 1e-12 * torch.sum(torch.abs(w * jv)).item()
