@@ -19,7 +19,11 @@ Checks:
   * the checkpointed gradient of the time-integrated kinetic energy of a
     Kelvin-Helmholtz run against central finite differences of the GPU
     forward with every stage's alpha frozen at the unperturbed run's values
-    (relative 1e-6).
+    (relative 1e-6);
+  * with the positivity floor: the RHS derivatives unchanged by it, and the
+    step derivatives through its density and pressure clamps (tangent
+    against central differences of the GPU step, relative 1e-5; the
+    dot-product identity).
 
 The 2D states are the Kelvin-Helmholtz state with a smooth perturbation of
 density, velocity and pressure.  The plain state is symmetric: along x its
