@@ -202,11 +202,6 @@ int ad_prepare(dgsem_handle* h) {
   const bool sc = c.volume_integral == DGSEM_VOLUME_SHOCK_CAPTURING;
   if (dgsem::uses_central_flux(c))
     return afail(h, DGSEM_E_UNSUPPORTED, "CentralFlux is not differentiated");
-  if (c.equation == DGSEM_NAVIER_STOKES)
-    for (int f = 0; f < 2 * c.dim; ++f)
-      if (c.face_boundary[f] != DGSEM_BC_PERIODIC)
-        return afail(h, DGSEM_E_UNSUPPORTED,
-                     "the Navier-Stokes tangent and adjoint are built for periodic meshes");
   const size_t cell = (size_t)h->E * h->nvars * h->nn;
   if (sc && !h->base.alpha_in) {
     // the indicator's alpha at the linearisation state, from a forward RHS

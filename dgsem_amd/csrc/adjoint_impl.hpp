@@ -173,6 +173,29 @@ __device__ void bc_outer_tangent(BcMap m, const double* dve, const int* dofs, co
   }
 }
 
+// Navier-Stokes faces of bounded meshes (parabolic_impl.hpp).  The gradient's
+// face value is the wall map of the own node's gradient variables
+// (wall_gradient_value: no slip zeroes the velocity, slip removes its normal
+// part; both linear in q and symmetric) or a Dirichlet state's, which is
+// constant; this applies the map's derivative (which is its transpose) to x.
+template <int DIM>
+__device__ __forceinline__ void wall_gradient_linear(int kind, const double* x, const double* nrm,
+                                                     double* y) {
+  if (kind == kBcDirichlet) {
+    for (int v = 0; v < DIM + 2; ++v) y[v] = 0.0;
+  } else {
+    wall_gradient_value<DIM>(kind, x, nrm, y);
+  }
+}
+
+// The variables [lo, hi) of the own node's viscous flux a boundary face
+// carries: all at a Dirichlet face, the momentum at a no-slip wall, none
+// otherwise.
+__device__ __forceinline__ void wall_viscous_range(int kind, int nv, int& lo, int& hi) {
+  lo = kind == kBcDirichlet ? 0 : 1;
+  hi = kind == kBcDirichlet ? nv : (kind == kBcNsNoSlipWall ? nv - 1 : 0);
+}
+
 // ---------------------------------------------------------------------------
 // Tangent
 // ---------------------------------------------------------------------------
@@ -215,6 +238,23 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) jvp_grad_kernel(
       }
     for (int side = 0; side < 2; ++side) {
       const int64_t nb = p.nbr[e * 2 * DIM + 2 * dir + side];
+      if constexpr (DIM == 2) {
+        if (nb < 0) {
+          // a boundary face: the wall map of the own node's variables
+          const int dof_own = side == 0 ? dofs[0] : dofs[n - 1];
+          const double sg = sign_j<NODAL, DIM, NN>(p, e, dof_own);
+          double x[NV], nrm[DIM], dqw[NV];
+          for (int v = 0; v < NV; ++v) x[v] = dq[v * NN + dof_own];
+          for (int c = 0; c < DIM; ++c) nrm[c] = sg * metric<NODAL, DIM, NN>(p, e, dir, c, dof_own);
+          wall_gradient_linear<DIM>(p.bc[2 * dir + side], x, nrm, dqw);
+          if (side == 0) {
+            for (int v = 0; v < NV; ++v) acc[0][v] -= sg * p.lhat0 * dqw[v];
+          } else {
+            for (int v = 0; v < NV; ++v) acc[n - 1][v] += sg * p.lhatN * dqw[v];
+          }
+          continue;
+        }
+      }
       const int dof_nb = side == 0 ? dofs[n - 1] : dofs[0];
       double s[NV], ds[NV], dqn[NV];
       load_state<DIM, NN>(p.u, nb, dof_nb, s);
@@ -308,12 +348,25 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) jvp_visc_kernel(
         for (int i = 0; i < n; ++i) r += p.Dhat[k][i] * f[v * NN + dofs[i]];
         acc[k][v] = r;
       }
-    {
-      const double sg = sign_j<NODAL, DIM, NN>(p, e, dofs[0]);
-      for (int v = 0; v < NV; ++v) acc[0][v] -= sg * f[v * NN + dofs[0]] * p.lhat0;
+    // lower face: this element is primary; at a boundary face only the
+    // variables the condition carries
+    int vlo = 0, vhi = NV;
+    if constexpr (DIM == 2) {
+      if (p.nbr[e * 2 * DIM + 2 * dir] < 0) wall_viscous_range(p.bc[2 * dir], NV, vlo, vhi);
     }
     {
-      const int64_t nb = p.nbr[e * 2 * DIM + 2 * dir + 1];
+      const double sg = sign_j<NODAL, DIM, NN>(p, e, dofs[0]);
+      for (int v = vlo; v < vhi; ++v) acc[0][v] -= sg * f[v * NN + dofs[0]] * p.lhat0;
+    }
+    const int64_t nb = p.nbr[e * 2 * DIM + 2 * dir + 1];
+    bool own_upper = false;  // a boundary upper face: the own node's flux
+    if constexpr (DIM == 2) own_upper = nb < 0;
+    if (own_upper) {
+      int ulo, uhi;
+      wall_viscous_range(p.bc[2 * dir + 1], NV, ulo, uhi);
+      const double sg = sign_j<NODAL, DIM, NN>(p, e, dofs[n - 1]);
+      for (int v = ulo; v < uhi; ++v) acc[n - 1][v] += sg * f[v * NN + dofs[n - 1]] * p.lhatN;
+    } else {
       double out[DIM][NV];
       contravariant_viscous_jvp<DIM, NN, NODAL>(a, nb, dofs[0], out);
       const double sg = sign_j<NODAL, DIM, NN>(p, nb, dofs[0]);
@@ -594,12 +647,30 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_visc1_kernel
     // the first node's value is lifted by this element (-l_hat0, lower face)
     // and by the lower neighbour (+l_hatN, its upper face, whose primary
     // element this is)
-    {
-      const int64_t lo = p.nbr[e * 2 * DIM + 2 * dir];
+    const int64_t lo = p.nbr[e * 2 * DIM + 2 * dir];
+    bool bnd_lo = false, bnd_hi = false;
+    if constexpr (DIM == 2) {
+      bnd_lo = lo < 0;
+      bnd_hi = p.nbr[e * 2 * DIM + 2 * dir + 1] < 0;
+    }
+    if (!bnd_lo) {
       double rl[NV];
       load_r<DIM, NN, NODAL>(a, lo, dofs[n - 1], rl);
       const double sg = sign_j<NODAL, DIM, NN>(p, e, dofs[0]);
       for (int v = 0; v < NV; ++v) hb[0][v] += sg * (p.lhatN * rl[v] - p.lhat0 * r[v * NN + dofs[0]]);
+    } else {
+      // a boundary lower face lifts the variables its condition carries
+      int vlo, vhi;
+      wall_viscous_range(p.bc[2 * dir], NV, vlo, vhi);
+      const double sg = sign_j<NODAL, DIM, NN>(p, e, dofs[0]);
+      for (int v = vlo; v < vhi; ++v) hb[0][v] -= sg * p.lhat0 * r[v * NN + dofs[0]];
+    }
+    if (bnd_hi) {
+      // a boundary upper face lifts the own last node's flux
+      int vlo, vhi;
+      wall_viscous_range(p.bc[2 * dir + 1], NV, vlo, vhi);
+      const double sg = sign_j<NODAL, DIM, NN>(p, e, dofs[n - 1]);
+      for (int v = vlo; v < vhi; ++v) hb[n - 1][v] += sg * p.lhatN * r[v * NN + dofs[n - 1]];
     }
     for (int k = 0; k < n; ++k)
       for (int v = 0; v < NV; ++v) s_hb[((slot * DIM + dir) * NV + v) * NN + dofs[k]] = hb[k][v];
@@ -668,13 +739,39 @@ __global__ void __launch_bounds__((EPW * Shape<DIM, N>::LINES)) vjp_visc2_kernel
     const int64_t lo = p.nbr[e * 2 * DIM + 2 * dir], hi = p.nbr[e * 2 * DIM + 2 * dir + 1];
     const double sg0 = sign_j<NODAL, DIM, NN>(p, e, dofs[0]);
     const double sgn = sign_j<NODAL, DIM, NN>(p, e, dofs[n - 1]);
-    const double sg_lo = sign_j<NODAL, DIM, NN>(p, lo, dofs[n - 1]);
-    const double sg_hi = sign_j<NODAL, DIM, NN>(p, hi, dofs[0]);
-    for (int v = 0; v < NV; ++v) {
-      const double g_lo = a.dgrad[((lo * DIM + dir) * NV + v) * NN + dofs[n - 1]];
-      const double g_hi = a.dgrad[((hi * DIM + dir) * NV + v) * NN + dofs[0]];
-      qb[0][v] += 0.5 * (sg_lo * p.lhatN * g_lo - sg0 * p.lhat0 * G[v * NN + dofs[0]]);
-      qb[n - 1][v] += 0.5 * (sgn * p.lhatN * G[v * NN + dofs[n - 1]] - sg_hi * p.lhat0 * g_hi);
+    bool bnd_lo = false, bnd_hi = false;
+    if constexpr (DIM == 2) {
+      bnd_lo = lo < 0;
+      bnd_hi = hi < 0;
+    }
+    if (!bnd_lo) {
+      const double sg_lo = sign_j<NODAL, DIM, NN>(p, lo, dofs[n - 1]);
+      for (int v = 0; v < NV; ++v) {
+        const double g_lo = a.dgrad[((lo * DIM + dir) * NV + v) * NN + dofs[n - 1]];
+        qb[0][v] += 0.5 * (sg_lo * p.lhatN * g_lo - sg0 * p.lhat0 * G[v * NN + dofs[0]]);
+      }
+    }
+    if (!bnd_hi) {
+      const double sg_hi = sign_j<NODAL, DIM, NN>(p, hi, dofs[0]);
+      for (int v = 0; v < NV; ++v) {
+        const double g_hi = a.dgrad[((hi * DIM + dir) * NV + v) * NN + dofs[0]];
+        qb[n - 1][v] += 0.5 * (sgn * p.lhatN * G[v * NN + dofs[n - 1]] - sg_hi * p.lhat0 * g_hi);
+      }
+    }
+    // a boundary face's value is the wall map of the own node's variables:
+    // its transpose (the map itself) takes the face term's cotangent back
+    if constexpr (DIM == 2) {
+      for (int side = 0; side < 2; ++side) {
+        if (side == 0 ? !bnd_lo : !bnd_hi) continue;
+        const int k = side == 0 ? 0 : n - 1;
+        const double sg = side == 0 ? sg0 : sgn;
+        double x[NV], nrm[DIM], y[NV];
+        for (int v = 0; v < NV; ++v)
+          x[v] = side == 0 ? -sg * p.lhat0 * G[v * NN + dofs[0]] : sg * p.lhatN * G[v * NN + dofs[n - 1]];
+        for (int c = 0; c < DIM; ++c) nrm[c] = sg * metric<NODAL, DIM, NN>(p, e, dir, c, dofs[k]);
+        wall_gradient_linear<DIM>(p.bc[2 * dir + side], x, nrm, y);
+        for (int v = 0; v < NV; ++v) qb[k][v] += y[v];
+      }
     }
   }
   for (int d = 0; d < DIM; ++d) {
@@ -1081,7 +1178,9 @@ hipError_t launch_gradient_t(const KParams& p, hipStream_t s) {
   constexpr int BLOCK = EPW * Shape<DIM, N>::LINES;
   const int64_t nb = (p.E + EPW - 1) / EPW;
   if (nb <= 0) return hipSuccess;
-  hipLaunchKernelGGL((gradient_kernel<DIM, N, EPW, NODAL, false>), dim3((unsigned)nb), dim3(BLOCK), 0, s, p);
+  // 2D meshes may have boundary faces (a neighbour entry of -1): the kernel's
+  // boundary branch forms their face values, as the derivative kernels do
+  hipLaunchKernelGGL((gradient_kernel<DIM, N, EPW, NODAL, DIM == 2>), dim3((unsigned)nb), dim3(BLOCK), 0, s, p);
   return hipGetLastError();
 }
 

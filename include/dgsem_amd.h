@@ -235,16 +235,15 @@ int dgsem_synchronize(dgsem_handle* h);
  * rounding, with J = dL/du at the state u: split form or shock capturing
  * (the blend (1 - alpha) split form + alpha FV with its subcell flux,
  * volume_flux.hpp:449-630, :821-869), LLF or HLLC surface and subcell
- * fluxes, Euler or Navier-Stokes (BR1); periodic meshes, and for Euler
- * also bounded 2D meshes (the outer states' dependence on the inner ones
- * included).  The
+ * fluxes, Euler or Navier-Stokes (BR1); periodic meshes, and bounded 2D
+ * meshes (the outer states' and the wall gradients' dependence on the inner
+ * ones included).  The
  * blending factor is not differentiated: it is held at the value the
  * forward evaluation at the same state computes (the indicator at u; at each
  * stage input for the step functions), or at the array set with
  * dgsem_freeze_alpha.  With the positivity floor the step functions pass
  * through each stage's clamp (its derivative at the stage's unclamped
- * result).  Navier-Stokes on bounded meshes is not differentiated
- * (DGSEM_E_UNSUPPORTED).
+ * result).
  * --------------------------------------------------------------------- */
 /* out = J(u) du */
 int dgsem_rhs_jvp(dgsem_handle* h, const double* u, const double* du, double* out);

@@ -1,12 +1,16 @@
-"""Tangent and adjoint of the Euler right-hand side on bounded 2D meshes:
-slip walls, outflow, Dirichlet states (constant and per node) and the mixed
+"""Tangent and adjoint of the right-hand side on bounded 2D meshes: slip
+walls, outflow, Dirichlet states (constant and per node) and the mixed
 Dirichlet/slip-wall face of the double Mach reflection, with the split form
-or shock capturing (alpha frozen), LLF or HLLC.  At a non-periodic face the
-flux takes the element's own trace and the outer state the boundary
-condition forms from it (boundary_set.hpp:88-111 order of faces): a slip
-wall mirrors the momentum about the face, outflow takes the mean of the
-line's nodes, a Dirichlet state is constant; the derivatives carry the
-outer state's dependence on the inner one.
+or shock capturing (alpha frozen), LLF or HLLC; and Navier-Stokes with its
+adiabatic no-slip and slip walls.  At a non-periodic face the flux takes the
+element's own trace and the outer state the boundary condition forms from it
+(boundary_set.hpp:88-111 order of faces): a slip wall mirrors the momentum
+about the face, outflow takes the mean of the line's nodes, a Dirichlet state
+is constant; the derivatives carry the outer state's dependence on the inner
+one.  Under Navier-Stokes the BR1 gradient's face value at a wall is the
+wall map of the own node's variables (navier_stokes_wall_boundary.hpp), and
+the viscous face flux is the own node's, for the variables the condition
+carries.
 
 The states drift (|v.n| > 0 on every face node, and not equal to the
 Dirichlet inflow's), and both velocity components carry node-wise noise: the
@@ -19,8 +23,7 @@ the max() have the same derivative.
 
 Bars: tangent against central differences (h = 1e-7) of the oracle's RHS,
 alpha overridden to the frozen values, relative 1e-6; <w, J v> = <J^T w, v>
-to 1e-12 relative, for the RHS and for one SSPRK3 step.  Navier-Stokes on
-bounded meshes is refused."""
+to 1e-12 relative, for the RHS and for one SSPRK3 step."""
 import numpy as np
 import pytest
 import torch
@@ -29,6 +32,8 @@ import dgsem_amd as dg
 from oracle import oracle as O
 from test_gpu_adjoint_sc import direction
 from test_gpu_boundaries import INFLOW, LAYOUTS
+from test_gpu_ns_walls import LAYOUTS as NS_LAYOUTS
+from test_gpu_ns_walls import make_ns_pair
 
 pytestmark = pytest.mark.gpu
 
@@ -153,16 +158,63 @@ def test_step_adjoint_dot_product(gpu, case):
     assert abs(lhs - rhs) <= 1e-12 * torch.sum(torch.abs(w * jv)).item()
 
 
-def test_navier_stokes_on_bounded_meshes_refused(gpu):
-    cfg = dg.SolverConfig(dim=2, polydeg=3, cells=(4, 4), domain_left=list(LEFT), domain_right=list(RIGHT),
-                          equation=dg.NAVIER_STOKES, mu=1e-3, prandtl=0.72,
-                          face_boundary=[dg.BC_NS_NOSLIP_WALL] * 4)
-    gs = dg.StructuredSolver(cfg)
-    u = gs.from_reference_layout(drifting_state(gs.node_coordinates(), 1))
-    out = torch.full_like(u, 7.0)
-    with pytest.raises(dg.DgsemError, match="periodic meshes"):
-        gs.rhs_jvp(u, torch.ones_like(u), out)
-    with pytest.raises(dg.DgsemError, match="periodic meshes"):
-        gs.rhs_vjp(u, torch.ones_like(u), out)
-    torch.cuda.synchronize()
-    assert bool((out == 7.0).all())
+NS_CASES = [
+    # layout, volume integral, geometry, N, cells
+    ("noslip_box", dg.SPLIT_FORM, dg.GEOMETRY_NODAL, 3, (6, 5)),
+    ("shock_tube", dg.SHOCK_CAPTURING, dg.GEOMETRY_NODAL, 3, (6, 5)),
+    ("euler_kinds", dg.SPLIT_FORM, dg.GEOMETRY_AFFINE, 2, (6, 5)),
+    ("mixed_channel", dg.SPLIT_FORM, dg.GEOMETRY_NODAL, 4, (5, 4)),
+    ("slip_channel", dg.SHOCK_CAPTURING, dg.GEOMETRY_AFFINE, 3, (6, 5)),
+]
+
+
+def ns_ids(c):
+    return (f"ns-{c[0]}-{'sc' if c[1] == dg.SHOCK_CAPTURING else 'split'}-"
+            f"{'nodal' if c[2] == dg.GEOMETRY_NODAL else 'affine'}-N{c[3]}")
+
+
+def ns_pair(case):
+    layout, volume, geometry, N, cells = case
+    oc, gs = make_ns_pair(N, cells, NS_LAYOUTS[layout], mu=0.02, geometry=geometry, volume=volume,
+                          left=LEFT, right=RIGHT)
+    return oc, gs, drifting_state(oc.coordinates(), seed=N)
+
+
+@pytest.mark.parametrize("case", NS_CASES, ids=[ns_ids(c) for c in NS_CASES])
+def test_navier_stokes_tangent_matches_oracle_finite_differences(gpu, case):
+    print(f"case bc ns jvp fd {ns_ids(case)}", flush=True)
+    oc, gs, u0 = ns_pair(case)
+    sc = case[1] == dg.SHOCK_CAPTURING
+    if sc:
+        _, alpha = oc.rhs(u0, with_alpha=True)
+        oc.set_alpha_override(alpha)
+    v = direction(u0, 4)
+    h = 1e-7
+    fd = (oc.rhs(u0 + h * v) - oc.rhs(u0 - h * v)) / (2 * h)
+    if sc:
+        oc.set_alpha_override(None)
+    u, vt = gs.from_reference_layout(u0), gs.from_reference_layout(v)
+    jv = gs.to_reference_layout(gs.rhs_jvp(u, vt))
+    err = float(np.linalg.norm(jv - fd) / np.linalg.norm(fd))
+    print(f"  rel {err:.2e}", flush=True)
+    assert err <= 1e-6
+
+
+@pytest.mark.parametrize("case", NS_CASES, ids=[ns_ids(c) for c in NS_CASES])
+def test_navier_stokes_adjoint_dot_product(gpu, case):
+    print(f"case bc ns vjp dot {ns_ids(case)}", flush=True)
+    _, gs, u0 = ns_pair(case)
+    u = gs.from_reference_layout(u0)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    v = torch.randn(gs.shape, dtype=torch.float64, device="cuda", generator=g)
+    w = torch.randn(gs.shape, dtype=torch.float64, device="cuda", generator=g)
+    jv = gs.rhs_jvp(u, v)
+    lhs = torch.sum(w * jv).item()
+    rhs = torch.sum(gs.rhs_vjp(u, w) * v).item()
+    scale = torch.sum(torch.abs(w * jv)).item()
+    print(f"  {abs(lhs - rhs) / scale:.2e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-12 * scale
+    u1, u2 = gs.ssprk3_stages(u, 1e-4)
+    jv = gs.step_jvp(1e-4, u, u1, u2, v)
+    lhs, rhs = torch.sum(w * jv).item(), torch.sum(gs.step_vjp(1e-4, u, u1, u2, w) * v).item()
+    assert abs(lhs - rhs) <= 1e-12 * torch.sum(torch.abs(w * jv)).item()
