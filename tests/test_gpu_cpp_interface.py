@@ -98,6 +98,52 @@ def test_cpp_double_mach_reflection_matches_oracle(gpu, tmp_path):
     assert np.linalg.norm(got - u) / np.linalg.norm(u) <= 1e-10
 
 
+def astro_jet_state(x, y, t, gamma=1.4):
+    if t > 0.0 and abs(x + 0.5) < 1e-8 and abs(y) < 0.05:
+        rho, vx = 5.0, 800.0
+    else:
+        rho, vx = 0.5, 0.0
+    p = 0.4127
+    return [rho, rho * vx, 0.0, p / (gamma - 1.0) + 0.5 * rho * vx * vx]
+
+
+@pytest.mark.gpu
+def test_cpp_astro_jet_matches_oracle(gpu, tmp_path):
+    """The jet inflow (examples/Euler2D/astro_jet): Dirichlet faces left and
+    right whose left state switches to the rho = 5, vx = 800 jet on
+    |y| < 0.05 for t > 0, periodic in y; the oracle re-evaluates the states
+    at each SSPRK3 stage time, so the first stage of the first step sees the
+    ambient state and every later one the jet."""
+    N, nx, ny, steps, dt = 3, 16, 16, 20, 1e-8
+    print(f"case cpp astro jet {nx}x{ny} steps={steps}", flush=True)
+    u0, got = run_example("astro_jet", tmp_path, nx, ny, steps, dt)
+    c = O.Case(2, N, (nx, ny), [-0.5, -0.5], [0.5, 0.5], periodic=[0, 1], volume=O.SHOCK_CAPTURING,
+               alpha_max=0.3, alpha_min=1e-4)
+    x = c.coordinates()
+    nodes = {f: boundary_nodes(c, f, N, (nx, ny)) for f in (0, 1)}
+
+    def set_time(t):
+        for f in (0, 1):
+            c.set_boundary(f, O.BC_DIRICHLET,
+                           values=np.array([astro_jet_state(*x[e, d], t) for e, d in nodes[f]]))
+
+    u = u0.reshape(c.shape).copy()
+    t = 0.0
+    for _ in range(steps):
+        set_time(t)
+        u1 = 1.0 * u + dt * c.rhs(u)
+        set_time(t + dt)
+        u2 = 3.0 / 4.0 * u + 1.0 / 4.0 * u1 + 1.0 / 4.0 * dt * c.rhs(u1)
+        set_time(t + 0.5 * dt)
+        u = 1.0 / 3.0 * u + 2.0 / 3.0 * u2 + 2.0 / 3.0 * dt * c.rhs(u2)
+        t += dt
+    got = got.reshape(u.shape)
+    assert np.isfinite(got).all()
+    # the jet has entered: the left column's momentum is no longer zero
+    assert np.abs(u[..., 1]).max() > 1.0
+    assert np.linalg.norm(got - u) / np.linalg.norm(u) <= 1e-10
+
+
 @pytest.mark.gpu
 def test_cpp_viscous_shock_tube_matches_oracle(gpu, tmp_path):
     """Navier-Stokes adiabatic walls (no slip on three sides, slip on top)
