@@ -99,7 +99,7 @@ def build(force: bool = True, verbose: bool = False, defines=(), out: str | None
 
 
 EXAMPLES = ["khi_shock_capturing", "double_mach_reflection", "viscous_shock_tube",
-            "taylor_green_observers", "astro_jet"]
+            "taylor_green_observers", "astro_jet", "riemann2d"]
 
 
 def build_examples() -> None:

@@ -145,6 +145,26 @@ def test_cpp_astro_jet_matches_oracle(gpu, tmp_path):
 
 
 @pytest.mark.gpu
+def test_cpp_riemann2d_matches_oracle(gpu, tmp_path):
+    """The four-quadrant Riemann problem (examples/Euler2D/riemann2d): outflow
+    on every face, N = 5, the example's CFL-0.2 time step."""
+    N, nx, ny, steps = 5, 20, 20, 30
+    dt = 0.2 * min(1.0 / nx, 1.0 / ny) / ((2 * (N + 1) - 1) * 2.0)
+    print(f"case cpp riemann2d {nx}x{ny} steps={steps}", flush=True)
+    u0, got = run_example("riemann2d", tmp_path, nx, ny, steps)
+    c = O.Case(2, N, (nx, ny), [0.0, 0.0], [1.0, 1.0], periodic=[0, 0], volume=O.SHOCK_CAPTURING,
+               alpha_max=0.5, alpha_min=0.001)
+    for f in range(4):
+        c.set_boundary(f, O.BC_OUTFLOW)
+    _, alpha = c.rhs(u0.reshape(c.shape), with_alpha=True)
+    assert 0 < int((alpha > 0).sum()) < c.E
+    ref = c.ssprk3(u0.reshape(c.shape), dt, steps)
+    got = got.reshape(ref.shape)
+    assert np.isfinite(got).all()
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-10
+
+
+@pytest.mark.gpu
 def test_cpp_viscous_shock_tube_matches_oracle(gpu, tmp_path):
     """Navier-Stokes adiabatic walls (no slip on three sides, slip on top)
     through the reference-style interface."""
