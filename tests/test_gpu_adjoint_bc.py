@@ -165,6 +165,7 @@ NS_CASES = [
     ("euler_kinds", dg.SPLIT_FORM, dg.GEOMETRY_AFFINE, 2, (6, 5)),
     ("mixed_channel", dg.SPLIT_FORM, dg.GEOMETRY_NODAL, 4, (5, 4)),
     ("slip_channel", dg.SHOCK_CAPTURING, dg.GEOMETRY_AFFINE, 3, (6, 5)),
+    ("shock_tube", dg.SPLIT_FORM, dg.GEOMETRY_NODAL, 7, (3, 3)),
 ]
 
 
