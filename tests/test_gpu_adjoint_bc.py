@@ -159,25 +159,27 @@ def test_step_adjoint_dot_product(gpu, case):
 
 
 NS_CASES = [
-    # layout, volume integral, geometry, N, cells
-    ("noslip_box", dg.SPLIT_FORM, dg.GEOMETRY_NODAL, 3, (6, 5)),
-    ("shock_tube", dg.SHOCK_CAPTURING, dg.GEOMETRY_NODAL, 3, (6, 5)),
-    ("euler_kinds", dg.SPLIT_FORM, dg.GEOMETRY_AFFINE, 2, (6, 5)),
-    ("mixed_channel", dg.SPLIT_FORM, dg.GEOMETRY_NODAL, 4, (5, 4)),
-    ("slip_channel", dg.SHOCK_CAPTURING, dg.GEOMETRY_AFFINE, 3, (6, 5)),
-    ("shock_tube", dg.SPLIT_FORM, dg.GEOMETRY_NODAL, 7, (3, 3)),
+    # layout, volume integral, geometry, N, cells, surface and subcell flux
+    ("noslip_box", dg.SPLIT_FORM, dg.GEOMETRY_NODAL, 3, (6, 5), L),
+    ("shock_tube", dg.SHOCK_CAPTURING, dg.GEOMETRY_NODAL, 3, (6, 5), L),
+    ("euler_kinds", dg.SPLIT_FORM, dg.GEOMETRY_AFFINE, 2, (6, 5), L),
+    ("mixed_channel", dg.SPLIT_FORM, dg.GEOMETRY_NODAL, 4, (5, 4), L),
+    ("slip_channel", dg.SHOCK_CAPTURING, dg.GEOMETRY_AFFINE, 3, (6, 5), L),
+    ("shock_tube", dg.SPLIT_FORM, dg.GEOMETRY_NODAL, 7, (3, 3), L),
+    ("shock_tube", dg.SHOCK_CAPTURING, dg.GEOMETRY_NODAL, 3, (6, 5), H),
+    ("euler_kinds", dg.SPLIT_FORM, dg.GEOMETRY_AFFINE, 3, (6, 5), H),
 ]
 
 
 def ns_ids(c):
     return (f"ns-{c[0]}-{'sc' if c[1] == dg.SHOCK_CAPTURING else 'split'}-"
-            f"{'nodal' if c[2] == dg.GEOMETRY_NODAL else 'affine'}-N{c[3]}")
+            f"{'nodal' if c[2] == dg.GEOMETRY_NODAL else 'affine'}-N{c[3]}-{'hllc' if c[5] == H else 'llf'}")
 
 
 def ns_pair(case):
-    layout, volume, geometry, N, cells = case
+    layout, volume, geometry, N, cells, flux = case
     oc, gs = make_ns_pair(N, cells, NS_LAYOUTS[layout], mu=0.02, geometry=geometry, volume=volume,
-                          left=LEFT, right=RIGHT)
+                          left=LEFT, right=RIGHT, surface=flux, fv=flux)
     return oc, gs, drifting_state(oc.coordinates(), seed=N)
 
 

@@ -30,12 +30,14 @@ STATE = [1.3, 0.2, -0.1, 2.4]
 
 
 def make_ns_pair(N, cells, faces, *, eq=dg.NAVIER_STOKES, mu=5e-3, geometry=dg.GEOMETRY_NODAL,
-                 volume=dg.SPLIT_FORM, left=(-1.0, -0.5), right=(1.0, 0.75), amax=0.5, amin=0.001):
+                 volume=dg.SPLIT_FORM, left=(-1.0, -0.5), right=(1.0, 0.75), amax=0.5, amin=0.001,
+                 surface=dg.FLUX_LLF, fv=dg.FLUX_LLF):
     periodic = [int(faces[0] == dg.BC_PERIODIC), int(faces[2] == dg.BC_PERIODIC)]
     if eq == dg.EULER:  # the Euler twin: walls become slip walls
         faces = [dg.BC_SLIP_WALL if k in (NOSLIP, SLIP) else k for k in faces]
     oc = O.Case(2, N, cells, list(left), list(right), periodic=periodic, eq=eq, mu=mu,
-                prandtl=PR, volume=volume, alpha_max=amax, alpha_min=amin)
+                prandtl=PR, volume=volume, alpha_max=amax, alpha_min=amin, surface_flux=surface,
+                fv_flux=fv)
     states = [STATE if k == dg.BC_DIRICHLET else None for k in faces]
     for f, kind in enumerate(faces):
         if kind != dg.BC_PERIODIC:
@@ -43,7 +45,7 @@ def make_ns_pair(N, cells, faces, *, eq=dg.NAVIER_STOKES, mu=5e-3, geometry=dg.G
     cfg = dg.SolverConfig(dim=2, polydeg=N, cells=cells, domain_left=list(left),
                           domain_right=list(right), equation=eq, mu=mu, prandtl=PR,
                           volume_integral=volume, alpha_max=amax, alpha_min=amin,
-                          geometry=geometry, face_boundary=faces,
+                          surface_flux=surface, fv_flux=fv, geometry=geometry, face_boundary=faces,
                           face_state=[s or [0.0] * 4 for s in states])
     gs = dg.StructuredSolver(cfg)
     if geometry == dg.GEOMETRY_AFFINE:
