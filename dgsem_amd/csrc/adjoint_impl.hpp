@@ -1,6 +1,10 @@
 // Tangent (JVP) and discrete adjoint (VJP) kernels of the DGSEM right-hand
-// side L(u) = -(1/J) (A(u) + R(u)): A the split-form volume integral plus the
-// LLF surface flux lifted with l_hat, R the BR1 viscous part (Navier-Stokes).
+// side L(u) = -(1/J) (A(u) + R(u)): A the split-form volume integral (under
+// shock capturing blended with the subcell finite volumes, alpha frozen)
+// plus the LLF or HLLC surface flux lifted with l_hat, R the BR1 viscous part
+// (Navier-Stokes).  On 2D meshes a face may be a boundary face (neighbour
+// entry -1): the outer state, the wall gradient value and the wall viscous
+// flux are functions of the own element's nodes, differentiated with them.
 // They differentiate exactly what rhs_kernel / gradient_kernel /
 // viscous_kernel compute, with the pointwise derivatives of physics_ad.hpp.
 //
