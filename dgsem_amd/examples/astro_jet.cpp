@@ -2,7 +2,7 @@
 // (examples/Euler2D/astro_jet/astro_jet.cpp) but against dgsem_hip.hpp:
 // Dirichlet faces left and right, periodic in y, N = 3, shock capturing with
 // alpha_max = 0.3, alpha_min = 1e-4.  The left face's state turns into the
-// Mach ~2000 jet (rho 5, vx 800) on |y| < 0.05 once t > 0, so the RHS at a
+// Mach ~2350 jet (rho 5, vx 800, p 0.4127) on |y| < 0.05 once t > 0, so the RHS at a
 // step's first stage sees the ambient state and the later stages the jet.
 // Usage: astro_jet nx ny steps dt out_prefix
 // writes out_prefix.u0.bin and out_prefix.u.bin, the initial and final
